@@ -1,0 +1,164 @@
+/*
+ * psx.h -- C ABI of libparticle_stabilizer_hip.so, the MI355X (gfx950) backend for the per-step
+ * sphere-settling hot path of particle-stabilizer.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status (PSX_OK == 0) and never
+ * calls exit(). The text of the last error on the calling thread is returned by psx_last_error().
+ *
+ * Reference interface each group replaces (paths relative to the reference repository):
+ *   - runSimulateCUDA(...)   include/ParticleStabilizerCUDA/PhysicsEngineCUDA.hpp:43-54, the dlopen'd
+ *     plugin entry (src/App/ParticleStabilizerMain.cpp:24-35,235-256). Its std::/glm signature is
+ *     kept by runSimulateHIP in psx_compat.hpp; the POD entry points below are what that shim and
+ *     every other caller (ctypes, the CLI, the tests) bind.
+ *   - runSimulate(...)       include/ParticleStabilizer/PhysicsEngine.hpp:31-42 (CPU backend, same
+ *     signature). Its step order (PhysicsEngine.cpp:38-48) is the semantics implemented here.
+ *
+ * Threading: a psx_sim handle is not thread-safe; separate handles are independent. All device work
+ * of a handle is ordered on one HIP stream (its own, or one injected with psx_set_stream).
+ */
+#ifndef PSX_H_
+#define PSX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSX_ABI_VERSION 1
+
+enum psx_status {
+  PSX_OK = 0,
+  PSX_ERR_INVALID = 1,  /* bad argument (null pointer, n <= 0, dt <= 0, min >= max, ...) */
+  PSX_ERR_HIP = 2,      /* a HIP runtime call failed */
+  PSX_ERR_ALLOC = 3,    /* device or host allocation failed */
+  PSX_ERR_STATE = 4,    /* call not valid in the handle's current state */
+  PSX_ERR_NUMERIC = 5,  /* non-finite particle state detected */
+  PSX_ERR_IO = 6        /* file could not be read or written */
+};
+
+enum psx_dtype { PSX_F32 = 0, PSX_F64 = 1 };
+
+/* Simulation constants. Mirrors the arguments of runSimulate / runSimulateCUDA
+ * (PhysicsEngine.hpp:31-42) and the CLI defaults of ParticleStabilizerMain.cpp:41-57. */
+typedef struct psx_params {
+  double gravity[3];
+  /* CoefficientOfRestitutionWall order (ContainerBase.hpp:10-34): zMin, zMax, xMin, xMax, yMin, yMax.
+   * With a mesh container walls[0] (zMin) is the mesh restitution (PhysicsEngine.cpp:107-109). */
+  double walls[6];
+  double spring;       /* coefficientOfSpring */
+  double restitution;  /* coefficientOfRestitutionSphere */
+  double dt;
+  double box_min[3];   /* box container; ignored when a mesh is given */
+  double box_max[3];
+  int32_t dtype;       /* enum psx_dtype: arithmetic type of the particle state on the device */
+  int32_t flags;       /* reserved, must be 0 */
+} psx_params;
+
+/* A triangle-soup container (PolygonContainer.cpp:8-90): n_triangles de-indexed triangles,
+ * vertices[9*t + 3*k + a] = coordinate a of corner k of triangle t. Normals follow the winding,
+ * normalize(cross(b - a, c - a)); particles are kept on the -normal side. */
+typedef struct psx_mesh {
+  int64_t n_triangles;
+  const double* vertices;
+} psx_mesh;
+
+/* One scene of a batch: its own constants, container and particle count. */
+typedef struct psx_scene {
+  psx_params params;
+  psx_mesh mesh;      /* n_triangles == 0: box container from params.box_min/max */
+  int64_t n;          /* number of particles of this scene (its slice of the batch arrays) */
+} psx_scene;
+
+/* Launch shapes, chosen by the autotuner (psx/tune.py) or left at 0 for the built-in defaults. */
+typedef struct psx_tuning {
+  int32_t block_integrate;  /* workgroup size of the integrate/hash/container kernel */
+  int32_t block_sort;       /* workgroup size of the scatter and rank/gather kernels */
+  int32_t block_pairs;      /* workgroup size of the pair-contact kernel */
+  int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
+} psx_tuning;
+
+typedef struct psx_sim psx_sim;
+
+/* ---- lifetime ------------------------------------------------------------------------------ */
+
+/* Builds a single-scene simulation (runSimulateCUDA's ParticleModelCUDA + container construction,
+ * PhysicsEngineCUDA.cpp:152-171). pos/vel are n x 3 (row-major), radius/mass are n; all host memory,
+ * copied. mesh may be NULL (box container). The grid follows ParticleBucket::initBucket
+ * (ParticleBucket.cpp:17-38) over the container bounds. */
+int psx_create(const psx_params* params, int64_t n, const double* pos, const double* vel,
+               const double* radius, const double* mass, const psx_mesh* mesh, psx_sim** out);
+
+/* Builds a batch of independent scenes advanced together, one launch per kernel per step.
+ * Particle arrays are the concatenation of the scenes' particles in scene order. All scenes share
+ * one dtype (scenes[0].params.dtype). No counterpart in the reference: the reference runs one
+ * scene per process. */
+int psx_batch_create(int32_t n_scenes, const psx_scene* scenes, const double* pos,
+                     const double* vel, const double* radius, const double* mass, psx_sim** out);
+
+int psx_destroy(psx_sim* sim);
+
+/* ---- stepping ------------------------------------------------------------------------------ */
+
+/* Enqueues nsteps steps (PhysicsEngine::startSimulationLoop's body, PhysicsEngine.cpp:38-48) on the
+ * handle's stream and returns without waiting. Statistics are not computed on this path. */
+int psx_step(psx_sim* sim, int64_t nsteps);
+
+int psx_synchronize(psx_sim* sim);
+
+/* Injects a caller-owned HIP stream (hipStream_t passed as void*); NULL restores the own stream. */
+int psx_set_stream(psx_sim* sim, void* hip_stream);
+
+int psx_set_tuning(psx_sim* sim, const psx_tuning* tuning);
+int psx_get_tuning(const psx_sim* sim, psx_tuning* tuning);
+
+/* ---- state --------------------------------------------------------------------------------- */
+
+/* Copies the state out in the caller's original particle order (runSimulate's copy-out,
+ * PhysicsEngine.cpp:148-159). Waits for the stream. Any of the pointers may be NULL. */
+int psx_get_state(psx_sim* sim, double* pos, double* vel);
+int psx_set_state(psx_sim* sim, const double* pos, const double* vel);
+
+/* Device-memory variants for PyTorch interop: n x 3 arrays of the handle's dtype, original order,
+ * stream-ordered on the handle's stream (no host wait). */
+int psx_get_state_device(psx_sim* sim, void* pos_dev, void* vel_dev);
+int psx_set_state_device(psx_sim* sim, const void* pos_dev, const void* vel_dev);
+
+int64_t psx_num_particles(const psx_sim* sim);
+int32_t psx_num_scenes(const psx_sim* sim);
+int64_t psx_num_cells(const psx_sim* sim);     /* total grid cells over all scenes */
+int32_t psx_dtype_of(const psx_sim* sim);
+
+/* Grid geometry of one scene: min corner, cell size h, cell counts (x, y, z), first global cell. */
+int psx_get_grid(const psx_sim* sim, int32_t scene, double* min3, double* h, int64_t* n3,
+                 int64_t* cell_base);
+
+/* ---- bucketing (ParticleModel::updateBuckets, ParticleModel.cpp:49-62) ------------------------- */
+
+/* Buckets the current positions without stepping and returns, in the caller's original order:
+ * cell[i] (linear cell id (ix*ny + iy)*nz + iz plus the scene's cell_base), count[c] and the
+ * exclusive scan offset[c] (C+1 entries, offset[C] == n), and perm[k] = particle id at sorted slot k
+ * (stable: ascending id inside a cell). Any output may be NULL. */
+int psx_debug_bucketing(psx_sim* sim, int32_t* cell, uint32_t* count, uint32_t* offset,
+                        int32_t* perm);
+
+/* ---- measurement --------------------------------------------------------------------------- */
+
+/* Runs nsteps steps with a HIP event pair around every kernel launch on the handle's stream and
+ * returns the mean device time per launch of each kernel (ms) in kernel_ms[0..*n_kernels). Kernel
+ * k's name is psx_kernel_name(k). Launches per step of each kernel go to launches_per_step. */
+int psx_profile(psx_sim* sim, int64_t nsteps, double* kernel_ms, double* launches_per_step,
+                int32_t max_kernels, int32_t* n_kernels);
+const char* psx_kernel_name(int32_t k);
+
+/* ---- errors / version ---------------------------------------------------------------------- */
+
+const char* psx_last_error(void);
+int psx_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSX_H_ */

@@ -1,0 +1,162 @@
+// psx_device.hpp -- device-side types and per-particle math of the HIP backend.
+//
+// Every arithmetic expression here is evaluated in the order the reference's CPU engine evaluates it
+// (glm's scalar definitions: dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, length = sqrt(dot),
+// normalize(v) = v * (1/sqrt(dot(v,v))), distance(p,q) = length(q-p)). Sources are compiled with
+// -ffp-contract=off and correctly rounded division and square root, so a kernel evaluated in
+// binary64 reproduces the reference's double results bit for bit, and in binary32 it reproduces the
+// same expression order rounded to float. The cell hash is always evaluated in binary64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psx {
+
+// 16-byte (float) / 32-byte (double) particle records: pos4 = (x, y, z, radius),
+// vel4 = (vx, vy, vz, mass). Two vector loads per particle.
+template <class T>
+struct alignas(4 * sizeof(T)) Vec4 {
+  T x, y, z, w;
+};
+
+template <class T>
+struct Vec3 {
+  T x, y, z;
+};
+
+// Uniform grid of one scene (ParticleBucket.cpp:17-38): min corner, cell size h (binary64),
+// cell counts and the scene's first cell in the global cell numbering.
+struct GridDev {
+  double min[3];
+  double h;
+  int32_t n[3];
+  int32_t cell_base;
+};
+
+// Per-scene constants on the device, in the state's arithmetic type T.
+template <class T>
+struct SceneDev {
+  GridDev grid;
+  T bmin[3], bmax[3];
+  // BoxContainer face coefficients (BoxContainer.cpp:14-19) by axis: emin[a] for the min face of
+  // axis a, emax[a] for the max face.
+  T emin[3], emax[3];
+  T gdt[3];  // gravity * dt, evaluated once in T (PhysicsEngine.cpp:24 evaluates the same product)
+  T dt;
+  T spring;
+  T es;      // sphere restitution
+  T ew;      // mesh restitution (walls.zMin, PhysicsEngine.cpp:107-109)
+  T rmax;    // largest radius of the scene
+  int32_t p_begin, p_end;  // the scene's slice of the (sorted) particle arrays
+  int32_t mesh;            // -1: box container; otherwise index into the mesh table
+  int32_t pad;
+};
+
+// Triangle of a mesh container: corners, normal and AABB (PolygonContainer.cpp:52-89).
+template <class T>
+struct TriDev {
+  T a[3], b[3], c[3], n[3], lo[3], hi[3];
+};
+
+// Triangle grid of one mesh: every cell lists, in ascending triangle order, each triangle that a
+// sphere of radius <= rmax centred in that cell can touch.
+struct MeshDev {
+  double min[3];
+  double h;
+  int32_t n[3];
+  int32_t tcell_base;  // first entry of this mesh in the global triangle-cell offset table
+};
+
+// Work assignment of one workgroup: its scene and slice of particles.
+struct BlockInfo {
+  int32_t scene;
+  int32_t p_start;
+  int32_t p_end;
+  int32_t pad;
+};
+
+// ParticleBucket::toBucketIndex{X,Y,Z} (ParticleBucket.cpp:66-106): (int64)((x - min)/h) clamped
+// to [0, n-1]. The cast follows x86-64 cvttsd2si, which the reference's build relies on: NaN and
+// |q| >= 2^63 give INT64_MIN and therefore clamp to 0.
+__device__ __host__ inline int32_t axis_index(double x, double mn, double h, int32_t n) {
+  const double q = (x - mn) / h;
+  if (!(q >= 1.0 && q < 9223372036854775808.0)) return 0;
+  if (q >= (double)n) return n - 1;
+  return (int32_t)q;
+}
+
+template <class T>
+__device__ __host__ inline T dot3(T ax, T ay, T az, T bx, T by, T bz) {
+  return (ax * bx + ay * by) + az * bz;
+}
+
+// PolygonContainer::getClosestPointOnPolygon and helpers (PolygonContainer.cpp:92-176).
+template <class T>
+__device__ inline Vec3<T> closest_on_segment(const Vec3<T>& p, const T* a, const T* b) {
+  const T abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+  const T apx = p.x - a[0], apy = p.y - a[1], apz = p.z - a[2];
+  T t = dot3(apx, apy, apz, abx, aby, abz) / dot3(abx, aby, abz, abx, aby, abz);
+  const T m = (t < T(1)) ? t : T(1);
+  t = (T(0) < m) ? m : T(0);
+  return {a[0] + abx * t, a[1] + aby * t, a[2] + abz * t};
+}
+
+template <class T>
+__device__ inline T dist_to(const Vec3<T>& p, const Vec3<T>& q) {
+  const T dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+  return sqrt(dot3(dx, dy, dz, dx, dy, dz));
+}
+
+template <class T>
+__device__ inline Vec3<T> closest_on_triangle(const TriDev<T>& t, const Vec3<T>& p) {
+  // projectPointOntoPlane(p, a, n)
+  const T vx = p.x - t.a[0], vy = p.y - t.a[1], vz = p.z - t.a[2];
+  const T ip = dot3(vx, vy, vz, t.n[0], t.n[1], t.n[2]);
+  Vec3<T> proj = {p.x - t.n[0] * ip, p.y - t.n[1] * ip, p.z - t.n[2] * ip};
+  // isPointOnPolygon(proj, a, b, c): v0 = c - a, v1 = b - a, v2 = proj - a
+  {
+    const T v0x = t.c[0] - t.a[0], v0y = t.c[1] - t.a[1], v0z = t.c[2] - t.a[2];
+    const T v1x = t.b[0] - t.a[0], v1y = t.b[1] - t.a[1], v1z = t.b[2] - t.a[2];
+    const T v2x = proj.x - t.a[0], v2y = proj.y - t.a[1], v2z = proj.z - t.a[2];
+    const T d00 = dot3(v0x, v0y, v0z, v0x, v0y, v0z);
+    const T d01 = dot3(v0x, v0y, v0z, v1x, v1y, v1z);
+    const T d02 = dot3(v0x, v0y, v0z, v2x, v2y, v2z);
+    const T d11 = dot3(v1x, v1y, v1z, v1x, v1y, v1z);
+    const T d12 = dot3(v1x, v1y, v1z, v2x, v2y, v2z);
+    const T inv = T(1) / (d00 * d11 - d01 * d01);
+    const T u = (d11 * d02 - d01 * d12) * inv;
+    const T w = (d00 * d12 - d01 * d02) * inv;
+    if ((u >= T(0)) && (w >= T(0)) && (u + w <= T(1))) return proj;
+  }
+  const Vec3<T> qab = closest_on_segment(p, t.a, t.b);
+  T best = dist_to(p, qab);
+  proj = qab;
+  const Vec3<T> qbc = closest_on_segment(p, t.b, t.c);
+  const T dbc = dist_to(p, qbc);
+  if (dbc < best) {
+    best = dbc;
+    proj = qbc;
+  }
+  const Vec3<T> qca = closest_on_segment(p, t.c, t.a);
+  const T dca = dist_to(p, qca);
+  if (dca < best) {
+    best = dca;
+    proj = qca;
+  }
+  return proj;
+}
+
+// Prefilter margin for the pair test: sqrt_rn(dd) < R can only hold when dd < R^2 (1 + margin).
+template <class T>
+struct PairMargin;
+template <>
+struct PairMargin<float> {
+  static constexpr float value = 1.0f + 1e-5f;
+};
+template <>
+struct PairMargin<double> {
+  static constexpr double value = 1.0 + 1e-12;
+};
+
+}  // namespace psx

@@ -1,0 +1,102 @@
+// psx_host.hpp -- host-side types of the HIP backend (scene description, the simulation interface
+// shared by the float and double implementations, per-kernel event timing).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "psx.h"
+
+namespace psx {
+
+constexpr int kNumKernels = 7;
+
+int set_error(int code, const std::string& msg);
+
+// Grid of ParticleBucket (ParticleBucket.cpp:17-27).
+struct HostGrid {
+  double min[3] = {0, 0, 0}, max[3] = {0, 0, 0};
+  double h = 0;
+  int64_t n[3] = {0, 0, 0};
+};
+HostGrid make_grid(double rmax, const double* mn, const double* mx);
+
+struct MeshHost {
+  int64_t nt = 0;
+  std::vector<double> v;  // 9 * nt de-indexed vertices
+  double min[3], max[3];  // container bounds (PolygonContainer.cpp:288-318)
+  // triangle grid
+  double th = 0;
+  int32_t tn[3] = {0, 0, 0};
+  std::vector<int32_t> off, list;
+};
+void build_tri_grid(MeshHost& m, double cell, double rmax, bool fp32);
+
+struct SceneHost {
+  psx_params prm;
+  int64_t n = 0, p_begin = 0;
+  int mesh = -1;
+  double rmax = 0;
+  HostGrid grid;
+  int64_t cell_base = 0;
+};
+
+inline psx_tuning default_tuning() {
+  psx_tuning t;
+  t.block_integrate = 256;
+  t.block_sort = 256;
+  t.block_pairs = 256;
+  t.use_graph = 0;
+  return t;
+}
+
+class Profiler {
+ public:
+  Profiler(int nk, int64_t nsteps);  // nsteps: number of steps that will be timed
+  ~Profiler();
+  void next_step();
+  hipEvent_t begin(int k);
+  hipEvent_t end(int k);
+  void collect(double* ms, double* per_step);
+
+ private:
+  int nk_;
+  int64_t step_ = 0;
+  std::vector<hipEvent_t> ev_;
+  std::vector<char> used_ = std::vector<char>(64, 0);
+};
+
+struct SimBase {
+  virtual ~SimBase() = default;
+  int64_t n = 0;
+  int64_t ncells = 0, cpad = 0;
+  int32_t ntiles = 0;
+  int64_t tri_count = 0, tcell_entries = 0;
+  std::vector<SceneHost> scenes;
+  std::vector<MeshHost> meshes;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  psx_tuning tuning = default_tuning();
+
+  virtual int32_t dtype() const = 0;
+  virtual void setup(const std::vector<double>& pos, const std::vector<double>& vel,
+                     const std::vector<double>& radius, const std::vector<double>& mass) = 0;
+  virtual void step(int64_t nsteps) = 0;
+  virtual void profile(int64_t nsteps, double* ms, double* per_step) = 0;
+  virtual void set_stream(hipStream_t s) = 0;
+  virtual void retune(const psx_tuning& t) = 0;
+  virtual void get_state(double* pos, double* vel) = 0;
+  virtual void set_state(const double* pos, const double* vel) = 0;
+  virtual void get_state_device(void* pos, void* vel) = 0;
+  virtual void set_state_device(const void* pos, const void* vel) = 0;
+  virtual void debug_bucketing(int32_t* cell, uint32_t* count, uint32_t* offset, int32_t* perm) = 0;
+  virtual uint32_t read_flags() = 0;
+};
+
+std::unique_ptr<SimBase> make_sim(int dtype);
+const char* kernel_name(int k);
+
+}  // namespace psx

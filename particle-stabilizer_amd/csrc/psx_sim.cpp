@@ -1,0 +1,962 @@
+// psx_sim.cpp -- host side of libparticle_stabilizer_hip.so: scene set-up (grid geometry, triangle
+// grid), device buffers, the step sequence, hipGraph replay, per-kernel event timing and the C ABI
+// declared in include/psx.h.
+//
+// A single simulation is a batch of one scene: every kernel takes a per-workgroup (scene, slice)
+// table, so plain and batched runs share the same launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "psx.h"
+#include "psx_kernels.hpp"
+#include "psx_host.hpp"
+
+namespace psx {
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PSX_HIP(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw Error(PSX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// grid geometry (ParticleBucket::initBucket, ParticleBucket.cpp:17-27)
+// ---------------------------------------------------------------------------------------------
+HostGrid make_grid(double rmax, const double* mn, const double* mx) {
+  HostGrid g;
+  const double coeff = (double)1.05f;  // COEFF_FOR_INTERVAL_OF_BUCKETS, ParticleBucket.hpp:17
+  g.h = rmax * 2.0 * coeff;
+  if (!(g.h > 0.0) || !std::isfinite(g.h))
+    throw Error(PSX_ERR_INVALID, "largest radius must be positive and finite");
+  for (int a = 0; a < 3; ++a) {
+    g.min[a] = mn[a];
+    g.max[a] = mx[a];
+    if (!(mx[a] > mn[a]) || !std::isfinite(mn[a]) || !std::isfinite(mx[a]))
+      throw Error(PSX_ERR_INVALID, "container bounds must satisfy min < max on every axis");
+    const double q = (mx[a] - mn[a]) / g.h;
+    if (!(q < 1.0e9)) throw Error(PSX_ERR_INVALID, "grid too large: container / cell size");
+    g.n[a] = (int64_t)q + 1;
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------------------------------------
+// triangle grid (replaces the O(N*T) candidate loop of PolygonContainer.cpp:244-278)
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct D3 {
+  double x, y, z;
+};
+inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 scale(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Distance from p to triangle abc by region classification (Ericson, Real-Time Collision Detection,
+// 5.1.5). Used only to decide which triangles a grid cell lists, with a safety margin.
+double point_triangle_distance(D3 p, D3 a, D3 b, D3 c) {
+  const D3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  const double d1 = dotd(ab, ap), d2 = dotd(ac, ap);
+  D3 q;
+  if (d1 <= 0 && d2 <= 0) {
+    q = a;
+  } else {
+    const D3 bp = sub(p, b);
+    const double d3 = dotd(ab, bp), d4 = dotd(ac, bp);
+    if (d3 >= 0 && d4 <= d3) {
+      q = b;
+    } else {
+      const double vc = d1 * d4 - d3 * d2;
+      if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        q = add(a, scale(ab, d1 / (d1 - d3)));
+      } else {
+        const D3 cp = sub(p, c);
+        const double d5 = dotd(ab, cp), d6 = dotd(ac, cp);
+        if (d6 >= 0 && d5 <= d6) {
+          q = c;
+        } else {
+          const double vb = d5 * d2 - d1 * d6;
+          if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+            q = add(a, scale(ac, d2 / (d2 - d6)));
+          } else {
+            const double va = d3 * d6 - d5 * d4;
+            if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+              q = add(b, scale(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+            } else {
+              const double den = va + vb + vc;
+              if (!(std::fabs(den) > 0)) {  // degenerate triangle: nearest of the three edges
+                auto seg = [&](D3 s0, D3 s1) {
+                  const D3 e = sub(s1, s0);
+                  const double ee = dotd(e, e);
+                  double t = ee > 0 ? dotd(sub(p, s0), e) / ee : 0.0;
+                  t = std::min(1.0, std::max(0.0, t));
+                  const D3 d = sub(p, add(s0, scale(e, t)));
+                  return std::sqrt(dotd(d, d));
+                };
+                return std::min(seg(a, b), std::min(seg(b, c), seg(c, a)));
+              }
+              const double v = vb / den, w = vc / den;
+              q = add(a, add(scale(ab, v), scale(ac, w)));
+            }
+          }
+        }
+      }
+    }
+  }
+  const D3 d = sub(p, q);
+  return std::sqrt(dotd(d, d));
+}
+
+}  // namespace
+
+// Builds the triangle grid of one mesh. A cell lists triangle t (ascending) when a sphere of radius
+// <= rmax whose centre hashes to that cell can be within contact distance of t:
+//   interior cells: distance(cell centre, t) <= half diagonal + rmax + margin;
+//   boundary cells (the hash clamps positions outside the domain into them, so they are unbounded
+//   on their outer side): t's AABB overlaps the cell box extended to infinity outward, grown by
+//   rmax + margin.
+// margin absorbs the rounding of the hash division and of the closest-point evaluation in T.
+void build_tri_grid(MeshHost& m, double cell, double rmax, bool fp32) {
+  const int64_t nt = m.nt;
+  for (int a = 0; a < 3; ++a) {
+    const double ext = m.max[a] - m.min[a];
+    m.tn[a] = (int32_t)std::max<int64_t>(1, (int64_t)(ext / cell) + 1);
+  }
+  // keep the table modest: coarsen until it has at most 8M cells
+  while ((int64_t)m.tn[0] * m.tn[1] * m.tn[2] > (int64_t)8 << 20) {
+    cell *= 1.25;
+    for (int a = 0; a < 3; ++a)
+      m.tn[a] = (int32_t)std::max<int64_t>(1, (int64_t)((m.max[a] - m.min[a]) / cell) + 1);
+  }
+  m.th = cell;
+  double lmax = 0.0;
+  for (int a = 0; a < 3; ++a) lmax = std::max(lmax, std::max(std::fabs(m.min[a]), std::fabs(m.max[a])));
+  const double margin = fp32 ? 1e-4 * (lmax + 1.0) + 1e-3 * rmax : 1e-9 * (lmax + 1.0);
+  const double half_diag = 0.5 * std::sqrt(3.0) * cell;
+  const double reach_in = half_diag + rmax + margin;  // interior sphere test radius
+  const double grow = rmax + margin;                  // boundary AABB test growth
+  const int64_t ncell = (int64_t)m.tn[0] * m.tn[1] * m.tn[2];
+  std::vector<std::vector<int32_t>> lists((size_t)ncell);
+  auto idx = [&](double x, int a) {
+    return (int64_t)axis_index(x, m.min[a], m.th, m.tn[a]);
+  };
+  for (int64_t t = 0; t < nt; ++t) {
+    const double* p = &m.v[9 * t];
+    const D3 A{p[0], p[1], p[2]}, B{p[3], p[4], p[5]}, C{p[6], p[7], p[8]};
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(p[a], std::min(p[3 + a], p[6 + a]));
+      hi[a] = std::max(p[a], std::max(p[3 + a], p[6 + a]));
+    }
+    const double reach = std::max(reach_in, grow + cell);
+    int64_t i0[3], i1[3];
+    for (int a = 0; a < 3; ++a) {
+      i0[a] = idx(lo[a] - reach, a);
+      i1[a] = idx(hi[a] + reach, a);
+    }
+    for (int64_t ix = i0[0]; ix <= i1[0]; ++ix)
+      for (int64_t iy = i0[1]; iy <= i1[1]; ++iy)
+        for (int64_t iz = i0[2]; iz <= i1[2]; ++iz) {
+          const int64_t ii[3] = {ix, iy, iz};
+          bool boundary = false;
+          for (int a = 0; a < 3; ++a) boundary |= (ii[a] == 0 || ii[a] == m.tn[a] - 1);
+          bool take;
+          if (boundary) {
+            take = true;
+            for (int a = 0; a < 3; ++a) {
+              const double clo = ii[a] == 0 ? -std::numeric_limits<double>::infinity()
+                                            : m.min[a] + (double)ii[a] * m.th;
+              const double chi = ii[a] == m.tn[a] - 1 ? std::numeric_limits<double>::infinity()
+                                                      : m.min[a] + (double)(ii[a] + 1) * m.th;
+              if (lo[a] > chi + grow || hi[a] < clo - grow) take = false;
+            }
+          } else {
+            const D3 ctr{m.min[0] + ((double)ix + 0.5) * m.th, m.min[1] + ((double)iy + 0.5) * m.th,
+                         m.min[2] + ((double)iz + 0.5) * m.th};
+            take = point_triangle_distance(ctr, A, B, C) <= reach_in;
+          }
+          if (take) lists[(size_t)((ix * m.tn[1] + iy) * m.tn[2] + iz)].push_back((int32_t)t);
+        }
+  }
+  m.off.assign((size_t)ncell + 1, 0);
+  m.list.clear();
+  for (int64_t c = 0; c < ncell; ++c) {
+    m.off[(size_t)c] = (int32_t)m.list.size();
+    m.list.insert(m.list.end(), lists[(size_t)c].begin(), lists[(size_t)c].end());
+  }
+  m.off[(size_t)ncell] = (int32_t)m.list.size();
+}
+
+// ---------------------------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    bytes = std::max<size_t>(b, 256);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(PSX_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(bytes) +
+                                     " bytes) failed: " + hipGetErrorString(e));
+    }
+  }
+  template <class U>
+  U* as() const {
+    return static_cast<U*>(p);
+  }
+};
+
+static const char* kKernelNames[kNumKernels] = {"k_integrate",   "k_scan_reduce",
+                                                "k_scan_tiles",  "k_scan_down",
+                                                "k_scatter_ids", "k_rank_gather",
+                                                "k_pairs"};
+
+// ---------------------------------------------------------------------------------------------
+// the simulation
+// ---------------------------------------------------------------------------------------------
+template <class T>
+struct SimImpl final : SimBase {
+  // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
+  // bucketing pass runs without the pair kernel
+  DevBuf pos[2], vel[2], id[2];
+  DevBuf cell_of, slot, tmp;
+  DevBuf count, offset, tile_sums, flags;
+  DevBuf scenes_d, meshes_d, tris_d, tcell_off_d, tcell_tri_d;
+  DevBuf io;  // 6n T staging for state transfer
+  std::map<int, std::unique_ptr<DevBuf>> btab;
+  std::map<int, int> btab_blocks;
+  int cur_pos = 0, cur_vel = 0;
+  hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  static constexpr int kGraphSteps = 16;  // even, so a chunk returns to the same buffer parity
+
+  ~SimImpl() override {
+    destroy_graphs();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+
+  void destroy_graphs() {
+    for (auto& row : graphs)
+      for (auto& g : row)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
+  }
+
+  int32_t dtype() const override { return sizeof(T) == 4 ? PSX_F32 : PSX_F64; }
+
+  const BlockInfo* table(int block, int* nblocks) {
+    auto it = btab.find(block);
+    if (it == btab.end()) {
+      std::vector<BlockInfo> t;
+      for (size_t s = 0; s < scenes.size(); ++s) {
+        const int64_t b = scenes[s].p_begin, e = b + scenes[s].n;
+        for (int64_t p = b; p < e; p += block)
+          t.push_back(BlockInfo{(int32_t)s, (int32_t)p, (int32_t)std::min<int64_t>(p + block, e), 0});
+      }
+      auto buf = std::make_unique<DevBuf>();
+      buf->alloc(t.size() * sizeof(BlockInfo));
+      PSX_HIP(hipMemcpy(buf->p, t.data(), t.size() * sizeof(BlockInfo), hipMemcpyHostToDevice));
+      btab_blocks[block] = (int)t.size();
+      it = btab.emplace(block, std::move(buf)).first;
+    }
+    *nblocks = btab_blocks[block];
+    return it->second->as<BlockInfo>();
+  }
+
+  void setup(const std::vector<double>& hpos, const std::vector<double>& hvel,
+             const std::vector<double>& hr, const std::vector<double>& hm) override {
+    const int64_t nn = n;
+    // particle arrays in the caller's order (which already groups scenes contiguously)
+    std::vector<Vec4<T>> p4((size_t)nn), v4((size_t)nn);
+    std::vector<int32_t> ids((size_t)nn);
+    for (int64_t i = 0; i < nn; ++i) {
+      p4[i] = Vec4<T>{(T)hpos[3 * i], (T)hpos[3 * i + 1], (T)hpos[3 * i + 2], (T)hr[i]};
+      v4[i] = Vec4<T>{(T)hvel[3 * i], (T)hvel[3 * i + 1], (T)hvel[3 * i + 2], (T)hm[i]};
+      ids[i] = (int32_t)i;
+    }
+    // scenes: r_max in T, grid, cell bases (ParticleModel::initBuckets, ParticleModel.cpp:28-47)
+    int64_t cells = 0;
+    std::vector<SceneDev<T>> sd(scenes.size());
+    std::vector<MeshDev> md;
+    std::vector<TriDev<T>> tris;
+    std::vector<int32_t> toff, ttri;
+    for (size_t s = 0; s < scenes.size(); ++s) {
+      SceneHost& sh = scenes[s];
+      double rmax = 0.0;
+      for (int64_t i = sh.p_begin; i < sh.p_begin + sh.n; ++i)
+        rmax = std::max(rmax, (double)(T)hr[(size_t)i]);
+      sh.rmax = rmax;
+      double mn[3], mx[3];
+      if (sh.mesh >= 0) {
+        const MeshHost& m = meshes[(size_t)sh.mesh];
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = m.min[a];
+          mx[a] = m.max[a];
+        }
+      } else {
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = sh.prm.box_min[a];
+          mx[a] = sh.prm.box_max[a];
+        }
+      }
+      sh.grid = make_grid(rmax, mn, mx);
+      sh.cell_base = cells;
+      const int64_t c = sh.grid.n[0] * sh.grid.n[1] * sh.grid.n[2];
+      if (c > ((int64_t)1 << 30)) throw Error(PSX_ERR_INVALID, "grid of a scene exceeds 2^30 cells");
+      cells += c;
+      if (cells > ((int64_t)1 << 30)) throw Error(PSX_ERR_INVALID, "total grid exceeds 2^30 cells");
+
+      SceneDev<T>& d = sd[s];
+      std::memset(&d, 0, sizeof(d));
+      for (int a = 0; a < 3; ++a) {
+        d.grid.min[a] = sh.grid.min[a];
+        d.grid.n[a] = (int32_t)sh.grid.n[a];
+        d.bmin[a] = (T)sh.prm.box_min[a];
+        d.bmax[a] = (T)sh.prm.box_max[a];
+        const T ga = (T)sh.prm.gravity[a];
+        const T dta = (T)sh.prm.dt;
+        d.gdt[a] = ga * dta;
+      }
+      d.grid.h = sh.grid.h;
+      d.grid.cell_base = (int32_t)sh.cell_base;
+      // walls: zMin, zMax, xMin, xMax, yMin, yMax
+      d.emin[0] = (T)sh.prm.walls[2];
+      d.emax[0] = (T)sh.prm.walls[3];
+      d.emin[1] = (T)sh.prm.walls[4];
+      d.emax[1] = (T)sh.prm.walls[5];
+      d.emin[2] = (T)sh.prm.walls[0];
+      d.emax[2] = (T)sh.prm.walls[1];
+      d.dt = (T)sh.prm.dt;
+      d.spring = (T)sh.prm.spring;
+      d.es = (T)sh.prm.restitution;
+      d.ew = (T)sh.prm.walls[0];
+      d.rmax = (T)rmax;
+      d.p_begin = (int32_t)sh.p_begin;
+      d.p_end = (int32_t)(sh.p_begin + sh.n);
+      d.mesh = -1;
+      if (sh.mesh >= 0) {
+        MeshHost& m = meshes[(size_t)sh.mesh];
+        build_tri_grid(m, 2.0 * sh.grid.h, rmax, sizeof(T) == 4);
+        MeshDev mdv;
+        std::memset(&mdv, 0, sizeof(mdv));
+        for (int a = 0; a < 3; ++a) {
+          mdv.min[a] = m.min[a];
+          mdv.n[a] = m.tn[a];
+        }
+        mdv.h = m.th;
+        mdv.tcell_base = (int32_t)toff.size();
+        const int32_t tri_base = (int32_t)tris.size();
+        for (size_t c2 = 0; c2 + 1 < m.off.size(); ++c2) toff.push_back((int32_t)ttri.size() + m.off[c2]);
+        for (int32_t t : m.list) ttri.push_back(tri_base + t);
+        // triangle records (PolygonContainer.cpp:52-89): normal in double, stored in T
+        for (int64_t t = 0; t < m.nt; ++t) {
+          const double* q = &m.v[9 * t];
+          const D3 A{q[0], q[1], q[2]}, B{q[3], q[4], q[5]}, C{q[6], q[7], q[8]};
+          const D3 e1 = sub(B, A), e2 = sub(C, A);
+          // glm::cross(x, y) and glm::normalize(v) = v * (1 / sqrt(dot(v, v)))
+          const D3 cr{e1.y * e2.z - e2.y * e1.z, e1.z * e2.x - e2.z * e1.x,
+                      e1.x * e2.y - e2.x * e1.y};
+          const double inv = 1.0 / std::sqrt((cr.x * cr.x + cr.y * cr.y) + cr.z * cr.z);
+          TriDev<T> td;
+          const double nrm[3] = {cr.x * inv, cr.y * inv, cr.z * inv};
+          for (int a = 0; a < 3; ++a) {
+            td.a[a] = (T)q[a];
+            td.b[a] = (T)q[3 + a];
+            td.c[a] = (T)q[6 + a];
+            td.n[a] = (T)nrm[a];
+            // glm::min(glm::min(a, b), c) / glm::max(glm::max(a, b), c) over the T corners
+            T lo = (td.b[a] < td.a[a]) ? td.b[a] : td.a[a];
+            lo = (td.c[a] < lo) ? td.c[a] : lo;
+            T hi = (td.a[a] < td.b[a]) ? td.b[a] : td.a[a];
+            hi = (hi < td.c[a]) ? td.c[a] : hi;
+            td.lo[a] = lo;
+            td.hi[a] = hi;
+          }
+          tris.push_back(td);
+        }
+        d.mesh = (int32_t)md.size();
+        md.push_back(mdv);
+      }
+    }
+    if (!toff.empty()) toff.push_back((int32_t)ttri.size());
+    ncells = cells;
+    cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
+    ntiles = (int32_t)(cpad / kScanTile);
+
+    // device allocations
+    for (int k = 0; k < 2; ++k) {
+      pos[k].alloc((size_t)nn * sizeof(Vec4<T>));
+      vel[k].alloc((size_t)nn * sizeof(Vec4<T>));
+      id[k].alloc((size_t)nn * sizeof(int32_t));
+    }
+    cell_of.alloc((size_t)nn * 4);
+    slot.alloc((size_t)nn * 4);
+    tmp.alloc((size_t)nn * 4);
+    count.alloc((size_t)cpad * 4);
+    offset.alloc(((size_t)cpad + 1) * 4);
+    tile_sums.alloc(((size_t)ntiles + 1) * 4);
+    flags.alloc(16);
+    io.alloc((size_t)nn * 6 * sizeof(T));
+    scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
+    meshes_d.alloc(std::max<size_t>(md.size(), 1) * sizeof(MeshDev));
+    tris_d.alloc(std::max<size_t>(tris.size(), 1) * sizeof(TriDev<T>));
+    tcell_off_d.alloc(std::max<size_t>(toff.size(), 1) * 4);
+    tcell_tri_d.alloc(std::max<size_t>(ttri.size(), 1) * 4);
+
+    PSX_HIP(hipMemcpy(pos[0].p, p4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
+    PSX_HIP(hipMemcpy(vel[0].p, v4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
+    PSX_HIP(hipMemcpy(id[0].p, ids.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
+    PSX_HIP(hipMemset(count.p, 0, count.bytes));
+    PSX_HIP(hipMemset(offset.p, 0, offset.bytes));
+    PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
+    PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
+    if (!md.empty())
+      PSX_HIP(hipMemcpy(meshes_d.p, md.data(), md.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
+    if (!tris.empty())
+      PSX_HIP(hipMemcpy(tris_d.p, tris.data(), tris.size() * sizeof(TriDev<T>), hipMemcpyHostToDevice));
+    if (!toff.empty())
+      PSX_HIP(hipMemcpy(tcell_off_d.p, toff.data(), toff.size() * 4, hipMemcpyHostToDevice));
+    if (!ttri.empty())
+      PSX_HIP(hipMemcpy(tcell_tri_d.p, ttri.data(), ttri.size() * 4, hipMemcpyHostToDevice));
+    tri_count = (int64_t)tris.size();
+    tcell_entries = (int64_t)ttri.size();
+    int nb;
+    table(tuning.block_integrate, &nb);
+    table(tuning.block_pairs, &nb);
+    PSX_HIP(hipDeviceSynchronize());
+  }
+
+  // One kernel launch, optionally bracketed by timing events.
+  template <class F>
+  void launch(int k, Profiler* prof, F&& f) {
+    if (prof) PSX_HIP(hipEventRecord(prof->begin(k), stream));
+    PSX_HIP(f());
+    if (prof) PSX_HIP(hipEventRecord(prof->end(k), stream));
+  }
+
+  void bucket_and_sort(int mode, Profiler* prof) {
+    const int a = cur_pos, vc = cur_vel;
+    int nb_i;
+    const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
+    const int32_t nn = (int32_t)n;
+    launch(0, prof, [&] {
+      return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
+                                 scenes_d.as<SceneDev<T>>(), pos[a].as<Vec4<T>>(),
+                                 vel[vc].as<Vec4<T>>(), cell_of.as<int32_t>(), slot.as<uint32_t>(),
+                                 count.as<uint32_t>(), meshes_d.as<MeshDev>(), tris_d.as<TriDev<T>>(),
+                                 tcell_off_d.as<int32_t>(), tcell_tri_d.as<int32_t>(),
+                                 flags.as<uint32_t>(), mode);
+    });
+    launch(1, prof, [&] {
+      return launch_scan_reduce(stream, count.as<uint32_t>(), tile_sums.as<uint32_t>(), ntiles);
+    });
+    launch(2, prof, [&] { return launch_scan_tiles(stream, tile_sums.as<uint32_t>(), ntiles); });
+    launch(3, prof, [&] {
+      return launch_scan_down(stream, count.as<uint32_t>(), offset.as<uint32_t>(),
+                              tile_sums.as<uint32_t>(), ntiles);
+    });
+    launch(4, prof, [&] {
+      return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
+                                slot.as<uint32_t>(), offset.as<uint32_t>(), id[a].as<int32_t>(),
+                                tmp.as<int32_t>());
+    });
+    launch(5, prof, [&] {
+      return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
+                                   offset.as<uint32_t>(), tmp.as<int32_t>(), pos[a].as<Vec4<T>>(),
+                                   vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(),
+                                   pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
+                                   id[1 - a].as<int32_t>());
+    });
+  }
+
+  void step_once(Profiler* prof) {
+    const int a = cur_pos, vc = cur_vel;
+    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof);
+    int nb_p;
+    const BlockInfo* bt_p = table(tuning.block_pairs, &nb_p);
+    launch(6, prof, [&] {
+      return launch_pairs<T>(stream, nb_p, tuning.block_pairs, bt_p, scenes_d.as<SceneDev<T>>(),
+                             pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
+                             offset.as<uint32_t>(), vel[vc].as<Vec4<T>>());
+    });
+    cur_pos = 1 - a;
+  }
+
+  void capture(int a, int vc) {
+    hipGraph_t g = nullptr;
+    PSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < kGraphSteps; ++k) step_once(nullptr);
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(stream, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      cur_pos = a;
+      cur_vel = vc;
+      throw;
+    }
+    PSX_HIP(hipStreamEndCapture(stream, &g));
+    hipError_t e = hipGraphInstantiate(&graphs[a][vc], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw Error(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    cur_pos = a;
+    cur_vel = vc;
+  }
+
+  void step(int64_t nsteps) override {
+    int64_t left = nsteps;
+    if (tuning.use_graph) {
+      while (left >= kGraphSteps) {
+        if (!graphs[cur_pos][cur_vel]) capture(cur_pos, cur_vel);
+        PSX_HIP(hipGraphLaunch(graphs[cur_pos][cur_vel], stream));
+        left -= kGraphSteps;
+      }
+    }
+    for (; left > 0; --left) step_once(nullptr);
+  }
+
+  void profile(int64_t nsteps, double* ms, double* per_step) override {
+    Profiler prof(kNumKernels, nsteps);
+    for (int64_t s = 0; s < nsteps; ++s) {
+      prof.next_step();
+      step_once(&prof);
+    }
+    PSX_HIP(hipStreamSynchronize(stream));
+    prof.collect(ms, per_step);
+  }
+
+  void set_stream(hipStream_t s) override {
+    PSX_HIP(hipStreamSynchronize(stream));
+    destroy_graphs();
+    stream = s ? s : own_stream;
+  }
+
+  void retune(const psx_tuning& t) override {
+    PSX_HIP(hipStreamSynchronize(stream));
+    destroy_graphs();
+    tuning = t;
+    int nb;
+    table(tuning.block_integrate, &nb);
+    table(tuning.block_pairs, &nb);
+  }
+
+  void get_state(double* hp, double* hv) override {
+    const int64_t nn = n;
+    T* d = io.as<T>();
+    PSX_HIP(launch_unsort<T>(stream, (int32_t)nn, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
+                             id[cur_pos].as<int32_t>(), d, d + 3 * nn));
+    std::vector<T> h((size_t)(6 * nn));
+    PSX_HIP(hipMemcpyAsync(h.data(), d, h.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+    for (int64_t k = 0; k < 3 * nn; ++k) {
+      if (hp) hp[k] = (double)h[(size_t)k];
+      if (hv) hv[k] = (double)h[(size_t)(3 * nn + k)];
+    }
+  }
+
+  void set_state(const double* hp, const double* hv) override {
+    const int64_t nn = n;
+    std::vector<T> h((size_t)(6 * nn));
+    for (int64_t k = 0; k < 3 * nn; ++k) {
+      h[(size_t)k] = hp ? (T)hp[k] : T(0);
+      h[(size_t)(3 * nn + k)] = hv ? (T)hv[k] : T(0);
+    }
+    T* d = io.as<T>();
+    PSX_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    PSX_HIP(launch_sort_in<T>(stream, (int32_t)nn, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
+                              id[cur_pos].as<int32_t>(), hp ? d : nullptr, hv ? d + 3 * nn : nullptr));
+    PSX_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+  }
+
+  void get_state_device(void* dp, void* dv) override {
+    PSX_HIP(launch_unsort<T>(stream, (int32_t)n, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
+                             id[cur_pos].as<int32_t>(), static_cast<T*>(dp), static_cast<T*>(dv)));
+  }
+
+  void set_state_device(const void* dp, const void* dv) override {
+    PSX_HIP(launch_sort_in<T>(stream, (int32_t)n, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
+                              id[cur_pos].as<int32_t>(), static_cast<const T*>(dp),
+                              static_cast<const T*>(dv)));
+  }
+
+  void debug_bucketing(int32_t* cell, uint32_t* cnt, uint32_t* off, int32_t* perm) override {
+    const int a = cur_pos, vc = cur_vel;
+    const int64_t nn = n;
+    bucket_and_sort(0, nullptr);
+    std::vector<int32_t> hc((size_t)nn), hid((size_t)nn), hperm((size_t)nn);
+    std::vector<uint32_t> hoff((size_t)ncells + 1);
+    PSX_HIP(hipMemcpyAsync(hc.data(), cell_of.p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipMemcpyAsync(hid.data(), id[a].p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipMemcpyAsync(hperm.data(), id[1 - a].p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipMemcpyAsync(hoff.data(), offset.p, ((size_t)ncells) * 4, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+    hoff[(size_t)ncells] = (uint32_t)nn;  // offset[C] == n (padding cells are empty)
+    cur_pos = 1 - a;
+    cur_vel = 1 - vc;
+    if (cell)
+      for (int64_t p = 0; p < nn; ++p) cell[hid[(size_t)p]] = hc[(size_t)p];
+    if (off) std::memcpy(off, hoff.data(), hoff.size() * 4);
+    if (cnt)
+      for (int64_t c = 0; c < ncells; ++c) cnt[c] = hoff[(size_t)c + 1] - hoff[(size_t)c];
+    if (perm) std::memcpy(perm, hperm.data(), (size_t)nn * 4);
+  }
+
+  uint32_t read_flags() override {
+    uint32_t f[4];
+    PSX_HIP(hipMemcpyAsync(f, flags.p, 16, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+    return f[0];
+  }
+};
+
+std::unique_ptr<SimBase> make_sim(int dtype) {
+  if (dtype == PSX_F32) return std::make_unique<SimImpl<float>>();
+  return std::make_unique<SimImpl<double>>();
+}
+
+const char* kernel_name(int k) {
+  if (k < 0 || k >= kNumKernels) return nullptr;
+  return kKernelNames[k];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Profiler
+// ---------------------------------------------------------------------------------------------
+Profiler::Profiler(int nk, int64_t nsteps) : nk_(nk) {
+  ev_.resize((size_t)(nk * std::max<int64_t>(nsteps, 1) * 2), nullptr);
+  for (auto& e : ev_) PSX_HIP(hipEventCreate(&e));
+}
+Profiler::~Profiler() {
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+void Profiler::next_step() { ++step_; }
+hipEvent_t Profiler::begin(int k) {
+  used_[k] = true;
+  return ev_[(size_t)(((step_ - 1) * nk_ + k) * 2)];
+}
+hipEvent_t Profiler::end(int k) { return ev_[(size_t)(((step_ - 1) * nk_ + k) * 2 + 1)]; }
+void Profiler::collect(double* ms, double* per_step) {
+  for (int k = 0; k < nk_; ++k) {
+    double tot = 0.0;
+    int64_t cnt = 0;
+    if (used_[k]) {
+      for (int64_t s = 0; s < step_; ++s) {
+        float t = 0.0f;
+        PSX_HIP(hipEventElapsedTime(&t, ev_[(size_t)((s * nk_ + k) * 2)], ev_[(size_t)((s * nk_ + k) * 2 + 1)]));
+        tot += t;
+        ++cnt;
+      }
+    }
+    ms[k] = cnt ? tot / (double)cnt : 0.0;
+    if (per_step) per_step[k] = used_[k] ? 1.0 : 0.0;
+  }
+}
+
+}  // namespace psx
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+using psx::Error;
+using psx::set_error;
+
+struct psx_sim {
+  std::unique_ptr<psx::SimBase> impl;
+};
+
+#define PSX_TRY(...)                                                          \
+  try {                                                                       \
+    __VA_ARGS__                                                               \
+  } catch (const Error& e) {                                                  \
+    return set_error(e.code, e.what());                                       \
+  } catch (const std::bad_alloc&) {                                           \
+    return set_error(PSX_ERR_ALLOC, "host allocation failed");                \
+  } catch (const std::exception& e) {                                         \
+    return set_error(PSX_ERR_INVALID, e.what());                              \
+  }
+
+static int check_params(const psx_params* p) {
+  if (!p) return set_error(PSX_ERR_INVALID, "params is NULL");
+  if (p->dtype != PSX_F32 && p->dtype != PSX_F64) return set_error(PSX_ERR_INVALID, "dtype must be PSX_F32 or PSX_F64");
+  if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return set_error(PSX_ERR_INVALID, "dt must be positive and finite");
+  if (p->flags != 0) return set_error(PSX_ERR_INVALID, "params.flags must be 0");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(p->gravity[a])) return set_error(PSX_ERR_INVALID, "gravity must be finite");
+  return PSX_OK;
+}
+
+static int create_impl(int32_t n_scenes, const psx_scene* scenes, const double* pos,
+                       const double* vel, const double* radius, const double* mass,
+                       psx_sim** out) {
+  if (!out) return set_error(PSX_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n_scenes <= 0 || !scenes) return set_error(PSX_ERR_INVALID, "need at least one scene");
+  if (!pos || !vel || !radius || !mass) return set_error(PSX_ERR_INVALID, "particle arrays must not be NULL");
+  const int32_t dtype = scenes[0].params.dtype;
+  int64_t total = 0;
+  for (int32_t s = 0; s < n_scenes; ++s) {
+    int rc = check_params(&scenes[s].params);
+    if (rc) return rc;
+    if (scenes[s].params.dtype != dtype) return set_error(PSX_ERR_INVALID, "all scenes must share one dtype");
+    if (scenes[s].n <= 0) return set_error(PSX_ERR_INVALID, "every scene needs at least one particle");
+    if (scenes[s].mesh.n_triangles < 0 || (scenes[s].mesh.n_triangles > 0 && !scenes[s].mesh.vertices))
+      return set_error(PSX_ERR_INVALID, "mesh vertices missing");
+    total += scenes[s].n;
+  }
+  if (total >= ((int64_t)1 << 31) - 1) return set_error(PSX_ERR_INVALID, "too many particles (>= 2^31)");
+  for (int64_t i = 0; i < total; ++i) {
+    if (!(radius[i] > 0.0) || !std::isfinite(radius[i]))
+      return set_error(PSX_ERR_INVALID, "radii must be positive and finite (particle " + std::to_string(i) + ")");
+    if (!(mass[i] > 0.0) || !std::isfinite(mass[i]))
+      return set_error(PSX_ERR_INVALID, "masses must be positive and finite (particle " + std::to_string(i) + ")");
+  }
+  PSX_TRY({
+    auto impl = psx::make_sim(dtype);
+    impl->n = total;
+    int64_t p0 = 0;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+      psx::SceneHost sh;
+      sh.prm = scenes[s].params;
+      sh.n = scenes[s].n;
+      sh.p_begin = p0;
+      p0 += sh.n;
+      if (scenes[s].mesh.n_triangles > 0) {
+        psx::MeshHost m;
+        m.nt = scenes[s].mesh.n_triangles;
+        m.v.assign(scenes[s].mesh.vertices, scenes[s].mesh.vertices + 9 * m.nt);
+        for (int a = 0; a < 3; ++a) {
+          m.min[a] = m.v[a];
+          m.max[a] = m.v[a];
+        }
+        // PolygonContainer::getMinCoords / getMaxCoords (PolygonContainer.cpp:288-318)
+        for (int64_t k = 0; k < 3 * m.nt; ++k)
+          for (int a = 0; a < 3; ++a) {
+            const double x = m.v[(size_t)(3 * k + a)];
+            if (!std::isfinite(x)) throw Error(PSX_ERR_INVALID, "mesh vertices must be finite");
+            m.min[a] = (x < m.min[a]) ? x : m.min[a];
+            m.max[a] = (m.max[a] < x) ? x : m.max[a];
+          }
+        sh.mesh = (int)impl->meshes.size();
+        impl->meshes.push_back(std::move(m));
+      } else {
+        for (int a = 0; a < 3; ++a)
+          if (!(sh.prm.box_max[a] > sh.prm.box_min[a]))
+            throw Error(PSX_ERR_INVALID, "box_min must be < box_max on every axis");
+      }
+      impl->scenes.push_back(sh);
+    }
+    hipStream_t st;
+    PSX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    impl->own_stream = st;
+    impl->stream = st;
+    std::vector<double> hp(pos, pos + 3 * total), hv(vel, vel + 3 * total), hr(radius, radius + total),
+        hm(mass, mass + total);
+    impl->setup(hp, hv, hr, hm);
+    auto* sim = new psx_sim;
+    sim->impl = std::move(impl);
+    *out = sim;
+    return PSX_OK;
+  })
+}
+
+extern "C" {
+
+int psx_create(const psx_params* params, int64_t n, const double* pos, const double* vel,
+               const double* radius, const double* mass, const psx_mesh* mesh, psx_sim** out) {
+  if (!params) return set_error(PSX_ERR_INVALID, "params is NULL");
+  if (n <= 0) return set_error(PSX_ERR_INVALID, "n must be positive");
+  psx_scene s;
+  std::memset(&s, 0, sizeof(s));
+  s.params = *params;
+  s.n = n;
+  if (mesh) s.mesh = *mesh;
+  return create_impl(1, &s, pos, vel, radius, mass, out);
+}
+
+int psx_batch_create(int32_t n_scenes, const psx_scene* scenes, const double* pos,
+                     const double* vel, const double* radius, const double* mass, psx_sim** out) {
+  return create_impl(n_scenes, scenes, pos, vel, radius, mass, out);
+}
+
+int psx_destroy(psx_sim* sim) {
+  if (!sim) return PSX_OK;
+  PSX_TRY({
+    if (sim->impl) (void)hipStreamSynchronize(sim->impl->stream);
+    delete sim;
+    return PSX_OK;
+  })
+}
+
+int psx_step(psx_sim* sim, int64_t nsteps) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  if (nsteps < 0) return set_error(PSX_ERR_INVALID, "nsteps must be >= 0");
+  PSX_TRY({
+    sim->impl->step(nsteps);
+    return PSX_OK;
+  })
+}
+
+int psx_synchronize(psx_sim* sim) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    const uint32_t f = sim->impl->read_flags();
+    if (f & psx::kFlagNonFinite)
+      return set_error(PSX_ERR_NUMERIC, "non-finite particle position or velocity produced by a step");
+    return PSX_OK;
+  })
+}
+
+int psx_set_stream(psx_sim* sim, void* hip_stream) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    sim->impl->set_stream(static_cast<hipStream_t>(hip_stream));
+    return PSX_OK;
+  })
+}
+
+int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
+  if (!sim || !t) return set_error(PSX_ERR_INVALID, "NULL argument");
+  psx_tuning c = *t;
+  psx_tuning d = psx::default_tuning();
+  if (c.block_integrate == 0) c.block_integrate = d.block_integrate;
+  if (c.block_sort == 0) c.block_sort = d.block_sort;
+  if (c.block_pairs == 0) c.block_pairs = d.block_pairs;
+  for (int b : {c.block_integrate, c.block_sort, c.block_pairs})
+    if (b < 64 || b > 1024 || (b % 64) != 0)
+      return set_error(PSX_ERR_INVALID, "block sizes must be multiples of 64 in [64, 1024]");
+  if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
+  PSX_TRY({
+    sim->impl->retune(c);
+    return PSX_OK;
+  })
+}
+
+int psx_get_tuning(const psx_sim* sim, psx_tuning* t) {
+  if (!sim || !t) return set_error(PSX_ERR_INVALID, "NULL argument");
+  *t = sim->impl->tuning;
+  return PSX_OK;
+}
+
+int psx_get_state(psx_sim* sim, double* pos, double* vel) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    sim->impl->get_state(pos, vel);
+    return PSX_OK;
+  })
+}
+
+int psx_set_state(psx_sim* sim, const double* pos, const double* vel) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  if (!pos || !vel) return set_error(PSX_ERR_INVALID, "pos and vel must not be NULL");
+  PSX_TRY({
+    sim->impl->set_state(pos, vel);
+    return PSX_OK;
+  })
+}
+
+int psx_get_state_device(psx_sim* sim, void* pos_dev, void* vel_dev) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    sim->impl->get_state_device(pos_dev, vel_dev);
+    return PSX_OK;
+  })
+}
+
+int psx_set_state_device(psx_sim* sim, const void* pos_dev, const void* vel_dev) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    sim->impl->set_state_device(pos_dev, vel_dev);
+    return PSX_OK;
+  })
+}
+
+int64_t psx_num_particles(const psx_sim* sim) { return sim ? sim->impl->n : -1; }
+int32_t psx_num_scenes(const psx_sim* sim) { return sim ? (int32_t)sim->impl->scenes.size() : -1; }
+int64_t psx_num_cells(const psx_sim* sim) { return sim ? sim->impl->ncells : -1; }
+int32_t psx_dtype_of(const psx_sim* sim) { return sim ? sim->impl->dtype() : -1; }
+
+int psx_get_grid(const psx_sim* sim, int32_t scene, double* min3, double* h, int64_t* n3,
+                 int64_t* cell_base) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  if (scene < 0 || scene >= (int32_t)sim->impl->scenes.size()) return set_error(PSX_ERR_INVALID, "scene out of range");
+  const psx::SceneHost& s = sim->impl->scenes[(size_t)scene];
+  for (int a = 0; a < 3; ++a) {
+    if (min3) min3[a] = s.grid.min[a];
+    if (n3) n3[a] = s.grid.n[a];
+  }
+  if (h) *h = s.grid.h;
+  if (cell_base) *cell_base = s.cell_base;
+  return PSX_OK;
+}
+
+int psx_debug_bucketing(psx_sim* sim, int32_t* cell, uint32_t* count, uint32_t* offset,
+                        int32_t* perm) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  PSX_TRY({
+    sim->impl->debug_bucketing(cell, count, offset, perm);
+    return PSX_OK;
+  })
+}
+
+int psx_profile(psx_sim* sim, int64_t nsteps, double* kernel_ms, double* launches_per_step,
+                int32_t max_kernels, int32_t* n_kernels) {
+  if (!sim || !kernel_ms || !n_kernels) return set_error(PSX_ERR_INVALID, "NULL argument");
+  if (nsteps <= 0 || nsteps > 100000) return set_error(PSX_ERR_INVALID, "nsteps must be in [1, 100000]");
+  if (max_kernels < psx::kNumKernels) return set_error(PSX_ERR_INVALID, "max_kernels too small");
+  PSX_TRY({
+    sim->impl->profile(nsteps, kernel_ms, launches_per_step);
+    *n_kernels = psx::kNumKernels;
+    return PSX_OK;
+  })
+}
+
+const char* psx_kernel_name(int32_t k) { return psx::kernel_name(k); }
+
+const char* psx_last_error(void) { return psx::g_last_error.c_str(); }
+
+int psx_abi_version(void) { return PSX_ABI_VERSION; }
+
+}  // extern "C"

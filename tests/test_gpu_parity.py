@@ -1,0 +1,119 @@
+"""Parity of the HIP backend with the oracle (CPU restatement of the reference's CPU step).
+
+Bar: bucketing (cell ids, counts, offsets, permutation) bit-exact; trajectories bit-exact against
+the oracle evaluated in the same scalar type (binary64 = the reference's arithmetic; binary32 = the
+same expression order in float).
+"""
+import numpy as np
+import pytest
+
+from helpers import (as_dtype, assert_bitwise, dense_box, hip_run, oracle_run, small_cylinder)
+
+pytestmark = pytest.mark.gpu
+
+
+def _bucket_compare(oracle, sim, pos, dtype, rmax, mn, mx):
+    got = sim.bucketing()
+    ref = oracle.bucketing(rmax, mn, mx, as_dtype(pos, dtype), dtype=dtype)
+    for k in ("cell", "count", "offset", "perm"):
+        assert_bitwise(got[k], ref[k], k)
+    g = sim.grid()
+    assert g["h"] == ref["h"] and np.array_equal(g["n"], ref["n"])
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_bucketing_box(psx_mod, oracle, dtype):
+    sc = dense_box(dtype=dtype)
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        rmax = float(as_dtype(sc.radius, dtype).max())
+        _bucket_compare(oracle, s, sc.pos, dtype, rmax, sc.params.box_min, sc.params.box_max)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_bucketing_edge_positions(psx_mod, oracle, dtype):
+    """Cell faces, outside the domain on every side, -0.0, huge and non-finite coordinates."""
+    rng = np.random.default_rng(7)
+    h_r = 0.5
+    from psx.sim import Params
+    prm = Params(box_min=(0.0, -2.0, 1.0), box_max=(8.0, 6.0, 9.0), dtype=dtype, dt=1e-4)
+    h, n3 = oracle.grid(h_r, prm.box_min, prm.box_max)
+    faces = []
+    for k in range(int(n3[0]) + 2):
+        faces.append([k * h, 1.0, 2.0])
+        faces.append([np.nextafter(k * h, -np.inf), 1.0, 2.0])
+    special = [[-0.0, -2.0, 1.0], [-1e-300, 0.0, 5.0], [-5.0, -7.0, -3.0], [50.0, 60.0, 70.0],
+               [1e30, -1e30, 4.0], [np.inf, 0.0, 2.0], [-np.inf, 1.0, 2.0], [np.nan, 1.0, 2.0],
+               [3.0, np.nan, np.nan], [8.0, 6.0, 9.0], [0.0, -2.0, 1.0]]
+    rand = rng.uniform(-3, 12, size=(400, 3))
+    pos = np.concatenate([np.asarray(faces), np.asarray(special), rand])
+    n = pos.shape[0]
+    r = np.full(n, h_r)
+    with psx_mod.Simulation(np.nan_to_num(pos), np.zeros((n, 3)), r, None, prm) as s:
+        s.set_state(pos, np.zeros((n, 3)))
+        _bucket_compare(oracle, s, pos, dtype, h_r, prm.box_min, prm.box_max)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("steps", [1, 60])
+def test_box_steps_bitexact(psx_mod, oracle, dtype, steps):
+    sc = dense_box(dtype=dtype)
+    ref_p, ref_v = oracle_run(oracle, sc, steps)
+    p, v = hip_run(sc, steps)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_box_long_bitexact(psx_mod, oracle, dtype):
+    """600 steps of a settling box at moderate stiffness."""
+    sc = dense_box(n_side=12, spacing=1.05, dtype=dtype, spring=300.0, vscale=0.5)
+    ref_p, ref_v = oracle_run(oracle, sc, 600)
+    p, v = hip_run(sc, 600)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_graph_replay_equals_eager(psx_mod, dtype):
+    sc = dense_box(dtype=dtype)
+    p0, v0 = hip_run(sc, 37, graph=False)
+    p1, v1 = hip_run(sc, 37, graph=True)
+    assert_bitwise(p1, p0, "pos")
+    assert_bitwise(v1, v0, "vel")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("steps", [1, 150])
+def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
+    """Triangle-grid mesh contact against the reference's brute-force O(N*T) loop."""
+    sc = small_cylinder(dtype=dtype)
+    ref_p, ref_v = oracle_run(oracle, sc, steps)
+    p, v = hip_run(sc, steps)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+def test_state_roundtrip(psx_mod):
+    sc = dense_box(dtype="f64")
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        s.step(3)
+        s.bucketing()
+        p, v = s.state()
+        s.set_state(p, v)
+        p2, v2 = s.state()
+        assert_bitwise(p2, p, "pos")
+        assert_bitwise(v2, v, "vel")
+
+
+def test_nonfinite_flag(psx_mod):
+    sc = dense_box(dtype="f64")
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        s.step(1)
+        s.synchronize()
+        p, v = s.state()
+        v[3, 0] = np.inf
+        s.set_state(p, v)
+        s.step(1)
+        with pytest.raises(psx_mod.PsxError) as e:
+            s.synchronize()
+        assert e.value.code == 5

@@ -1,0 +1,363 @@
+"""Pins the oracle (CPU restatement of the reference's CPU step) without the reference binary.
+
+The reference has no tests, fixtures or golden vectors for this path, and its CPU engine cannot be
+built here (glm, spdlog, assimp and SimView come from an empty submodule). Every oracle function is
+therefore pinned by what the mathematics fixes: known grid answers, closed forms (free fall, wall
+reflection, two-body restitution and spring kicks, plane and crease contact), invariants
+(momentum of the restitution term, bucketing = stable counting sort), brute force (O(N^2) pair
+enumeration, dense sampling of a triangle) and a textbook routine (Ericson's closest point).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from psx import meshes, scenes
+from psx.sim import Params, mass_of
+
+
+# ---------------------------------------------------------------------------------------------
+# grid geometry and cell hash (ParticleBucket.cpp:17-38, 66-106)
+# ---------------------------------------------------------------------------------------------
+def test_grid_known_answers(oracle):
+    # h = r_max * 2 * (double)1.05f
+    h, n = oracle.grid(0.55, (0, 0, 0), (20, 20, 60))
+    assert h == 0.55 * 2.0 * float(np.float32(1.05))
+    assert abs(h - 1.1549999475) < 1e-9 and list(n) == [18, 18, 52]
+    h, n = oracle.grid(0.8, (-40, -40, 0), (40, 40, 300))
+    assert abs(h - 1.6799999237) < 1e-9 and list(n) == [48, 48, 179]
+    tri = meshes.hopper().reshape(-1, 3)
+    h, n = oracle.grid(0.8, tri.min(0), tri.max(0))
+    assert list(n) == [179, 179, 358]
+
+
+def test_grid_of_generated_workloads(oracle):
+    c1 = scenes.box_settle()
+    _, n = oracle.grid(float(c1.radius.max()), c1.params.box_min, c1.params.box_max)
+    assert list(n) == [18, 18, 52]
+    c2 = scenes.cylinder_settle()
+    v = c2.mesh.reshape(-1, 3)
+    h, n = oracle.grid(float(np.float32(c2.radius.max())), v.min(0), v.max(0))
+    assert list(n) == [48, 48, 179] and c2.mesh.shape[0] == 2176
+
+
+def test_axis_index_special_cases(oracle):
+    h = 0.5  # dyadic: (x - min)/h is exact for dyadic x
+    ai = oracle.axis_index
+    assert ai(0.0, 0.0, h, 10) == 0
+    assert ai(-0.0, 0.0, h, 10) == 0
+    assert ai(0.5, 0.0, h, 10) == 1          # exactly on a face -> upper cell
+    assert ai(np.nextafter(0.5, 0), 0.0, h, 10) == 0
+    assert ai(4.75, 0.0, h, 10) == 9
+    assert ai(5.0, 0.0, h, 10) == 9          # at max -> clamped to n-1
+    assert ai(1e9, 0.0, h, 10) == 9
+    assert ai(-0.25, 0.0, h, 10) == 0        # truncation toward zero
+    assert ai(-3.0, 0.0, h, 10) == 0         # clamp below
+    assert ai(float("nan"), 0.0, h, 10) == 0
+    # x86-64 cvttsd2si gives INT64_MIN for |q| >= 2^63 and for inf: the clamp then yields 0
+    assert ai(float("inf"), 0.0, h, 10) == 0
+    assert ai(1e300, 0.0, h, 10) == 0
+    assert ai(-float("inf"), 0.0, h, 10) == 0
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_bucketing_is_stable_counting_sort(oracle, dtype):
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-2, 14, size=(3000, 3))
+    res = oracle.bucketing(0.5, (0, 0, 0), (10, 11, 12), pos, dtype=dtype)
+    cell, count, offset, perm = res["cell"], res["count"], res["offset"], res["perm"]
+    ncell = int(np.prod(res["n"]))
+    assert count.shape == (ncell,) and offset.shape == (ncell + 1,)
+    assert np.array_equal(count, np.bincount(cell, minlength=ncell))
+    assert np.array_equal(offset[:-1], np.concatenate([[0], np.cumsum(count)[:-1]]))
+    assert offset[-1] == len(pos)
+    assert np.array_equal(perm, np.argsort(cell, kind="stable"))
+    # z-fastest linear ids, each coordinate clamped into the grid
+    ix, rem = np.divmod(cell, res["n"][1] * res["n"][2])
+    iy, iz = np.divmod(rem, res["n"][2])
+    assert (ix < res["n"][0]).all() and (iy < res["n"][1]).all() and (iz < res["n"][2]).all()
+    inside = ((pos >= 0) & (pos < [10, 11, 12])).all(1)
+    h = res["h"]
+    p = pos.astype(np.float32).astype(np.float64) if dtype == "f32" else pos
+    assert np.array_equal(ix[inside], np.floor(p[inside, 0] / h).astype(int))
+
+
+# ---------------------------------------------------------------------------------------------
+# integrate (PhysicsEngine.cpp:19-26): free-fall closed form, exact for dyadic inputs
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_free_fall_closed_form(oracle, dtype):
+    dt, g = 2.0 ** -8, (0.0, 0.0, -8.0)
+    x0 = np.array([[10.0, 10.0, 50.0]])
+    v0 = np.array([[1.0, -2.0, 0.5]])
+    prm = Params(gravity=g, dt=dt, box_min=(-100, -100, -100), box_max=(100, 100, 100),
+                 spring=0.0, restitution=0.0)
+    for n in (1, 7, 64):
+        p, v = oracle.run(prm.as_array(), x0, v0, [0.5], [1.0], n, dtype=dtype)
+        xn = x0 + n * dt * v0 + np.asarray(g) * dt * dt * n * (n - 1) / 2
+        vn = v0 + n * np.asarray(g) * dt
+        assert np.array_equal(p.astype(np.float64), xn) and np.array_equal(v.astype(np.float64), vn)
+
+
+# ---------------------------------------------------------------------------------------------
+# box walls (BoxContainer.cpp:22-65): per face, distinct coefficients
+# ---------------------------------------------------------------------------------------------
+WALLS = (0.11, 0.12, 0.13, 0.14, 0.15, 0.16)  # zMin, zMax, xMin, xMax, yMin, yMax
+
+
+@pytest.mark.parametrize("axis,side,coeff", [(0, 0, 0.13), (0, 1, 0.14), (1, 0, 0.15),
+                                             (1, 1, 0.16), (2, 0, 0.11), (2, 1, 0.12)])
+def test_wall_face_closed_form(oracle, axis, side, coeff):
+    lo, hi = np.zeros(3), np.array([4.0, 5.0, 6.0])
+    x0 = np.array([2.0, 2.5, 3.0])
+    v0 = np.array([0.25, -0.5, 0.75])
+    r = 0.5
+    x0[axis] = lo[axis] + 0.25 if side == 0 else hi[axis] - 0.25   # penetrating this face only
+    prm = Params(gravity=(0, 0, 0), walls=WALLS, dt=2.0 ** -20, box_min=lo, box_max=hi,
+                 spring=0.0, restitution=0.0)
+    p, v = oracle.run(prm.as_array(), x0[None], v0[None], [r], [1.0], 1)
+    x1 = x0 + v0 * 2.0 ** -20
+    exp_x, exp_v = x1.copy(), v0.copy()
+    exp_x[axis] = lo[axis] + r if side == 0 else hi[axis] - r
+    exp_v[axis] = -v0[axis] * coeff
+    assert np.array_equal(p[0], exp_x) and np.array_equal(v[0], exp_v)
+
+
+def test_wall_both_faces_max_applied_last(oracle):
+    prm = Params(gravity=(0, 0, 0), walls=WALLS, dt=2.0 ** -20, box_min=(0, 0, 0),
+                 box_max=(1.0, 10.0, 10.0), spring=0.0, restitution=0.0)
+    x0 = np.array([[0.5, 5.0, 5.0]])
+    v0 = np.array([[2.0, 0.0, 0.0]])
+    p, v = oracle.run(prm.as_array(), x0, v0, [0.75], [1.0], 1)
+    assert p[0, 0] == 1.0 - 0.75
+    assert v[0, 0] == -(-2.0 * 0.13) * 0.14
+
+
+# ---------------------------------------------------------------------------------------------
+# pair contact (ParticleModel.cpp:64-144): two-body closed forms
+# ---------------------------------------------------------------------------------------------
+def _two_body(oracle, es, k, m=(1.0, 1.0), v1=(1.0, 0.3, 0.0), v2=(-1.0, 0.0, 0.2)):
+    prm = Params(gravity=(0, 0, 0), dt=2.0 ** -40, box_min=(0, 0, 0), box_max=(10, 10, 10),
+                 spring=k, restitution=es)
+    x = np.array([[4.6, 5.0, 5.0], [5.3, 5.1, 5.05]])
+    v = np.array([v1, v2], float)
+    p, vn = oracle.run(prm.as_array(), x, v, [0.5, 0.5], list(m), 1)
+    d = p[0] - p[1]
+    nrm = d / np.linalg.norm(d)
+    return p, v, vn, nrm, d
+
+
+def test_restitution_half_equal_masses_stops_normal_motion(oracle):
+    p, v, vn, nrm, _ = _two_body(oracle, es=0.5, k=0.0)
+    rel0 = (v[0] - v[1]) @ nrm
+    assert abs(rel0) > 0.5
+    assert abs((vn[0] - vn[1]) @ nrm) < 1e-13
+    # tangential relative velocity untouched
+    t0 = (v[0] - v[1]) - rel0 * nrm
+    t1 = (vn[0] - vn[1]) - ((vn[0] - vn[1]) @ nrm) * nrm
+    assert np.allclose(t0, t1, atol=1e-13)
+
+
+def test_restitution_one_reverses_normal_motion(oracle):
+    p, v, vn, nrm, _ = _two_body(oracle, es=1.0, k=0.0)
+    assert abs((vn[0] - vn[1]) @ nrm + (v[0] - v[1]) @ nrm) < 1e-13
+
+
+def test_spring_kick_along_normal(oracle):
+    k = 1234.5
+    p, v, vn, nrm, d = _two_body(oracle, es=0.0, k=k)
+    delta = 1.0 - np.linalg.norm(d)
+    assert delta > 0
+    assert np.allclose(vn[0] - v[0], k * delta * nrm, rtol=1e-13, atol=1e-12)
+    assert np.allclose(vn[1] - v[1], -k * delta * nrm, rtol=1e-13, atol=1e-12)
+
+
+def test_restitution_conserves_momentum_unequal_masses(oracle):
+    m = (0.7, 2.9)
+    p, v, vn, nrm, _ = _two_body(oracle, es=0.8, k=0.0, m=m)
+    dp = m[0] * (vn[0] - v[0]) + m[1] * (vn[1] - v[1])
+    assert np.abs(dp).max() < 1e-12
+    assert np.abs(vn[0] - v[0]).max() > 0.1
+
+
+def test_no_contact_no_kick(oracle):
+    prm = Params(gravity=(0, 0, 0), dt=2.0 ** -40, box_min=(0, 0, 0), box_max=(10, 10, 10),
+                 spring=1e4, restitution=0.5)
+    x = np.array([[4.0, 5.0, 5.0], [5.0000001, 5.0, 5.0]])  # distance just above r1 + r2
+    v = np.array([[1.0, 0, 0], [-1.0, 0, 0]])
+    _, vn = oracle.run(prm.as_array(), x, v, [0.5, 0.5], [1.0, 1.0], 1)
+    assert np.array_equal(vn, v)
+
+
+def _kick(xi, xj, vi, vj, ri, rj, mi, mj, k, es):
+    """One term of the velocity kick in the reference's expression order, Python floats."""
+    d = [xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]]
+    dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+    ln = math.sqrt(dd)
+    ov = ri + rj - ln
+    dv = [vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]]
+    s = (2.0 * mj / (mi + mj)) * (((dv[0] * d[0] + dv[1] * d[1]) + dv[2] * d[2]) / dd)
+    inv = 1.0 / math.sqrt(dd)
+    return [(-((s * d[c]) * es)) + (k * ov) * (d[c] * inv) for c in range(3)]
+
+
+def test_pairs_equal_bruteforce_in_grid_order(oracle):
+    """O(N^2) enumeration finds exactly the grid's contacts, summed in the grid's order."""
+    rng = np.random.default_rng(11)
+    n = 260
+    pos = rng.uniform(2.0, 8.0, size=(n, 3))
+    vel = rng.normal(0, 1, size=(n, 3))
+    r = rng.uniform(0.3, 0.6, n)
+    m = mass_of(r)
+    k, es = 777.0, 0.6
+    prm = Params(gravity=(0, 0, 0), dt=2.0 ** -60, box_min=(0, 0, 0), box_max=(10, 10, 10),
+                 spring=k, restitution=es)
+    p1, v1 = oracle.run(prm.as_array(), pos, vel, r, m, 1)
+    # positions after integrate (no walls are hit)
+    assert ((pos - r[:, None] > 0) & (pos + r[:, None] < 10)).all()
+    x = [list(map(float, row)) for row in p1]
+    h, n3 = oracle.grid(float(r.max()), (0, 0, 0), (10, 10, 10))
+    cells = np.clip(np.floor(p1 / h).astype(int), 0, n3 - 1)
+    expected = []
+    contacts = 0
+    for i in range(n):
+        terms = []
+        for j in range(n):
+            if j == i:
+                continue
+            e = [x[j][c] - x[i][c] for c in range(3)]
+            if math.sqrt((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]) < r[i] + r[j]:
+                dc = cells[j] - cells[i]
+                assert (np.abs(dc) <= 1).all()          # completeness of the 27-cell search
+                order = (dc[0] + 1) * 9 + (dc[1] + 1) * 3 + (dc[2] + 1)
+                terms.append((order, j))
+        contacts += len(terms)
+        acc = list(map(float, vel[i]))
+        for _, j in sorted(terms):
+            t = _kick(x[i], x[j], vel[i], vel[j], float(r[i]), float(r[j]), float(m[i]),
+                      float(m[j]), k, es)
+            acc = [acc[c] + t[c] for c in range(3)]
+        expected.append(acc)
+    assert contacts > 100
+    assert np.array_equal(v1, np.asarray(expected))
+
+
+# ---------------------------------------------------------------------------------------------
+# mesh (PolygonContainer.cpp:52-286)
+# ---------------------------------------------------------------------------------------------
+def _ericson(p, a, b, c):
+    """Closest point on triangle abc to p (Ericson, Real-Time Collision Detection, 5.1.5)."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + d1 / (d1 - d3) * ab
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + d2 / (d2 - d6) * ac
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (d4 - d3) / ((d4 - d3) + (d5 - d6)) * (c - b)
+    den = 1.0 / (va + vb + vc)
+    return a + ab * (vb * den) + ac * (vc * den)
+
+
+def test_closest_point_matches_textbook(oracle):
+    rng = np.random.default_rng(5)
+    for _ in range(400):
+        a, b, c = rng.normal(size=(3, 3)) * rng.uniform(0.2, 3)
+        p = rng.normal(size=3) * 2
+        q = oracle.closest_point(a, b, c, p)
+        qe = _ericson(p, a, b, c)
+        assert abs(np.linalg.norm(q - p) - np.linalg.norm(qe - p)) < 1e-9 * (1 + np.linalg.norm(p))
+        assert np.linalg.norm(q - qe) < 1e-6 * (1 + np.linalg.norm(p))
+
+
+def test_closest_point_dense_sampling(oracle):
+    a, b, c = np.array([0.0, 0, 0]), np.array([2.0, 0.2, 0]), np.array([0.3, 1.5, 0.4])
+    u, v = np.meshgrid(np.linspace(0, 1, 301), np.linspace(0, 1, 301))
+    keep = u + v <= 1
+    samples = a + u[keep, None] * (b - a) + v[keep, None] * (c - a)
+    rng = np.random.default_rng(6)
+    for p in rng.uniform(-2, 3, size=(40, 3)):
+        q = oracle.closest_point(a, b, c, p)
+        dmin = np.linalg.norm(samples - p, axis=1).min()
+        assert np.linalg.norm(q - p) <= dmin + 1e-12
+        assert np.linalg.norm(q - p) >= dmin - 0.02
+
+
+def test_generated_meshes_have_outward_normals(oracle):
+    cyl = meshes.open_cylinder()
+    n = oracle.mesh_normals(cyl)
+    ctr = cyl.reshape(-1, 3, 3).mean(1)
+    side = np.abs(n[:, 2]) < 0.5
+    rad = ctr[side, :2] / np.linalg.norm(ctr[side, :2], axis=1, keepdims=True)
+    assert side.sum() == 2048 and ((n[side, :2] * rad).sum(1) > 0.99).all()
+    assert np.allclose(n[~side], [0, 0, -1])
+    box = meshes.box_mesh()
+    nb = oracle.mesh_normals(box)
+    cb = box.reshape(-1, 3, 3).mean(1) - np.array([20.0, 20.0, 30.0])
+    assert box.shape[0] == 80 and ((nb * cb).sum(1) > 0).all()
+    hop = meshes.hopper()
+    nh = oracle.mesh_normals(hop)
+    ch = hop.reshape(-1, 3, 3).mean(1)
+    cone = (ch[:, 2] > 1e-9) & (ch[:, 2] < 199.0)
+    radial = (nh[cone, :2] * ch[cone, :2]).sum(1)
+    assert hop.shape[0] == 8192 and (radial > 0).all() and (nh[cone, 2] < 0).all()
+    assert np.allclose(oracle.mesh_normals(hop[-256:]), [0, 0, -1])
+
+
+def _floor(size=10.0):
+    a, b, c, d = (np.array(x, float) for x in ([-size, -size, 0], [size, -size, 0],
+                                               [size, size, 0], [-size, size, 0]))
+    # winding seen from below: normal -z
+    return np.array([np.concatenate([a, c, b]), np.concatenate([a, d, c])])
+
+
+def test_mesh_floor_plane_closed_form(oracle):
+    tri = _floor()
+    assert np.allclose(oracle.mesh_normals(tri), [0, 0, -1])
+    e = 0.37
+    prm = Params(gravity=(0, 0, 0), walls=(e,) * 6, dt=2.0 ** -30, spring=0.0, restitution=0.0)
+    x0 = np.array([[3.0, -4.0, 0.25]])        # inside the first triangle only, penetrating
+    v0 = np.array([[1.0, 0.5, -2.0]])
+    p, v = oracle.run(prm.as_array(), x0, v0, [0.5], [1.0], 1, mesh=tri)
+    assert p[0, 2] == 0.5
+    assert np.allclose(p[0, :2], x0[0, :2] + v0[0, :2] * 2.0 ** -30, rtol=0, atol=1e-15)
+    assert np.allclose(v[0], e * np.array([1.0, 0.5, 2.0]), rtol=1e-15)
+
+
+def test_mesh_crease_averages_reflections(oracle):
+    e = 0.5
+    floor = _floor()
+    # wall x = 0 facing -x (particles live at x > 0)
+    a, b, c, d = (np.array(x, float) for x in ([0, -10, -1], [0, 10, -1], [0, 10, 10], [0, -10, 10]))
+    wall = np.array([np.concatenate([a, c, b]), np.concatenate([a, d, c])])
+    nw = oracle.mesh_normals(wall)
+    assert np.allclose(nw, [-1, 0, 0])
+    tri = np.concatenate([floor, wall])
+    prm = Params(gravity=(0, 0, 0), walls=(e,) * 6, dt=2.0 ** -40, spring=0.0, restitution=0.0)
+    x0 = np.array([[0.3, 2.0, 0.4]])
+    v0 = np.array([[-1.0, 0.25, -3.0]])
+    p, v = oracle.run(prm.as_array(), x0, v0, [0.5], [1.0], 1, mesh=tri)
+    rf = v0[0] * [1, 1, -1]
+    rw = v0[0] * [-1, 1, 1]
+    assert np.allclose(v[0], e * (rf + rw) / 2, rtol=1e-14)
+    assert np.allclose(p[0], [0.5, 2.0, 0.5], atol=1e-9)
+
+
+def test_mass_formula(oracle):
+    for r in (0.3, 0.5, 1.0, 2.0, 5.0):
+        m = oracle.mass(1.0, r)
+        assert math.isclose(m, 4.0 / 3.0 * math.pi * r ** 3, rel_tol=4e-16)
+        assert m == float(mass_of([r])[0])
