@@ -152,6 +152,27 @@ int psx_profile(psx_sim* sim, int64_t nsteps, double* kernel_ms, double* launche
                 int32_t max_kernels, int32_t* n_kernels);
 const char* psx_kernel_name(int32_t k);
 
+/* ---- host file formats (no GPU involved) --------------------------------------------------- */
+
+/* ParticleData::readFromCSV (ParticleIO.cpp:90-144): header line skipped, tokens split on ',',
+ * empty and "null" tokens dropped, lines without exactly 4 tokens reported on stderr and skipped
+ * (counted in *n_skipped), x,y,z,r parsed with std::stod; velocity3 is every particle's initial
+ * velocity, mass = density*4/3*pi*r^3. Arrays are malloc'd (release with psx_free). A missing file
+ * returns PSX_ERR_IO (the reference exits). */
+int psx_read_csv(const char* path, double density, const double* velocity3, int64_t* n,
+                 double** pos, double** vel, double** radius, double** mass, int64_t* n_skipped);
+
+/* ParticleData::writeAsCSV (ParticleIO.cpp:58-88): header "x,y,z,radius", each value in fixed
+ * notation with 10 - len(str(int(|v|))) - [v < 0] - 1 decimals; parent directories created. */
+int psx_write_csv(const char* path, int64_t n, const double* pos, const double* radius);
+
+/* Triangle container from a Wavefront OBJ file (v / f records; polygons split as a fan from their
+ * first corner), returned de-indexed in face order as PolygonContainer::readFromFile builds it
+ * (PolygonContainer.cpp:381-417). *vertices holds 9 * *n_triangles doubles (psx_free). */
+int psx_read_obj(const char* path, int64_t* n_triangles, double** vertices);
+
+void psx_free(void* p);
+
 /* ---- errors / version ---------------------------------------------------------------------- */
 
 const char* psx_last_error(void);

@@ -66,7 +66,8 @@ def test_box_steps_bitexact(psx_mod, oracle, dtype, steps):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_box_long_bitexact(psx_mod, oracle, dtype):
     """600 steps of a settling box at moderate stiffness."""
-    sc = dense_box(n_side=12, spacing=1.05, dtype=dtype, spring=300.0, vscale=0.5)
+    sc = dense_box(n_side=12, spacing=1.05, dtype=dtype, spring=300.0, vscale=0.5,
+                   box=(14.0, 14.0, 16.0))
     ref_p, ref_v = oracle_run(oracle, sc, 600)
     p, v = hip_run(sc, 600)
     assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
