@@ -49,23 +49,22 @@ struct SceneDev {
   T ew;      // mesh restitution (walls.zMin, PhysicsEngine.cpp:107-109)
   T rmax;    // largest radius of the scene
   int32_t p_begin, p_end;  // the scene's slice of the (sorted) particle arrays
-  int32_t mesh;            // -1: box container; otherwise index into the mesh table
-  int32_t pad;
+  int32_t mesh;            // -1: box container; otherwise the scene has a triangle container
+  int32_t tcell_base;      // first entry of the scene's cells in the triangle-cell offset table
 };
 
-// Triangle of a mesh container: corners, normal and AABB (PolygonContainer.cpp:52-89).
+// Triangle of a mesh container: corners and normal (PolygonContainer.cpp:52-68).
 template <class T>
-struct TriDev {
-  T a[3], b[3], c[3], n[3], lo[3], hi[3];
+struct TriGeo {
+  T a[3], b[3], c[3], n[3];
 };
 
-// Triangle grid of one mesh: every cell lists, in ascending triangle order, each triangle that a
-// sphere of radius <= rmax centred in that cell can touch.
-struct MeshDev {
-  double min[3];
-  double h;
-  int32_t n[3];
-  int32_t tcell_base;  // first entry of this mesh in the global triangle-cell offset table
+// One candidate of a grid cell's triangle list: the triangle's AABB (PolygonContainer.cpp:70-89)
+// stored inline, so the reference's AABB reject costs one load, and the triangle id.
+template <class T>
+struct alignas(sizeof(T) == 4 ? 32 : 64) TriBox {
+  T lo[3], hi[3];
+  int32_t tri;
 };
 
 // Work assignment of one workgroup: its scene and slice of particles.
@@ -109,7 +108,7 @@ __device__ inline T dist_to(const Vec3<T>& p, const Vec3<T>& q) {
 }
 
 template <class T>
-__device__ inline Vec3<T> closest_on_triangle(const TriDev<T>& t, const Vec3<T>& p) {
+__device__ inline Vec3<T> closest_on_triangle(const TriGeo<T>& t, const Vec3<T>& p) {
   // projectPointOntoPlane(p, a, n)
   const T vx = p.x - t.a[0], vy = p.y - t.a[1], vz = p.z - t.a[2];
   const T ip = dot3(vx, vy, vz, t.n[0], t.n[1], t.n[2]);

@@ -23,17 +23,16 @@ struct HostGrid {
   int64_t n[3] = {0, 0, 0};
 };
 HostGrid make_grid(double rmax, const double* mn, const double* mx);
+struct MeshHost;
+void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32);
 
 struct MeshHost {
   int64_t nt = 0;
   std::vector<double> v;  // 9 * nt de-indexed vertices
   double min[3], max[3];  // container bounds (PolygonContainer.cpp:288-318)
-  // triangle grid
-  double th = 0;
-  int32_t tn[3] = {0, 0, 0};
+  // triangle lists per cell of the scene's particle grid
   std::vector<int32_t> off, list;
 };
-void build_tri_grid(MeshHost& m, double cell, double rmax, bool fp32);
 
 struct SceneHost {
   psx_params prm;
@@ -50,6 +49,7 @@ inline psx_tuning default_tuning() {
   t.block_sort = 256;
   t.block_pairs = 256;
   t.use_graph = 0;
+  t.pair_variant = 0;
   return t;
 }
 

@@ -5,33 +5,45 @@
 //   k_integrate    x += v*dt; v += g*dt; cell hash of the new x; box walls or mesh contact;
 //                  per-cell count (atomic, returns the particle's arrival slot)
 //   k_scan_reduce  \
-//   k_scan_tiles    > exclusive scan of the per-cell counts into offsets; clears the counts
-//   k_scan_down    /
+//   k_scan_tiles    > exclusive scan of the per-cell counts into offsets (tile sums, tile
+//   k_scan_down    /  prefixes, each tile rescanned with wave64 scans); clears the counts
 //   k_scatter_ids  ids to offset[cell] + arrival slot (an unordered cell list)
 //   k_rank_gather  rank of each particle among its cell mates by id (stable order) and gather of
 //                  the state into cell-sorted order
 //   k_pairs        27-cell spring/restitution velocity kick, Jacobi over the sorted snapshot
 // Particle state lives in cell-sorted order between steps; an id array carries original indices.
+//
+// Workgroups of the per-particle kernels are numbered XCD-major: the GPU deals workgroups to its 8
+// XCDs round-robin, so workgroup b is given tile (b % 8) * ceil(tiles / 8) + b / 8, and each XCD
+// (with its own L2) sweeps one contiguous eighth of the cell-sorted particles, where neighbour
+// columns are close in memory.
 #include "psx_kernels.hpp"
 
 namespace psx {
 
 // ---------------------------------------------------------------------------------------------
-// wave64 helpers
+// wave64 and launch helpers
 // ---------------------------------------------------------------------------------------------
+constexpr int kXcds = 8;
+
+__host__ __device__ inline int32_t xcd_grid(int32_t tiles) {
+  return ((tiles + kXcds - 1) / kXcds) * kXcds;
+}
+
+__device__ inline int32_t xcd_tile(int32_t b, int32_t tiles) {
+  const int32_t per = (tiles + kXcds - 1) / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const uint32_t t = __shfl_up(v, d, 64);
     if (lane >= d) v += t;
   }
-  return v;
-}
-
-__device__ inline uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
 
@@ -73,115 +85,180 @@ __device__ inline void box_walls(const SceneDev<T>* __restrict__ s, Vec4<T>& x, 
 }
 
 // PolygonContainer::resolveCollisions (PolygonContainer.cpp:178-286) for one particle. Instead of
-// all T triangles, only the particle's triangle-grid cell list is visited; it holds, in ascending
-// order, every triangle the particle can touch, and the triangles it omits are exactly ones whose
-// contact test fails (they change nothing), so the sequential position update and the velocity sum
-// see the same triangles in the same order as the full loop.
+// all T triangles, only the list of the particle's grid cell is visited; it holds, in ascending
+// order, every triangle a particle hashed to that cell can touch, and the triangles it omits are
+// exactly ones whose contact test fails (they change nothing), so the sequential position update
+// and the velocity sum see the same triangles in the same order as the full loop.
+//
+// Contact state of one particle: the running position y, the velocity sum and the hit count.
 template <class T>
-__device__ inline void mesh_contact(const SceneDev<T>* __restrict__ s, const MeshDev* __restrict__ meshes,
-                                    const TriDev<T>* __restrict__ tris,
-                                    const int32_t* __restrict__ tcell_off,
-                                    const int32_t* __restrict__ tcell_tri, Vec4<T>& x, Vec4<T>& v) {
-  const MeshDev& m = meshes[s->mesh];
-  const int32_t tx = axis_index((double)x.x, m.min[0], m.h, m.n[0]);
-  const int32_t ty = axis_index((double)x.y, m.min[1], m.h, m.n[1]);
-  const int32_t tz = axis_index((double)x.z, m.min[2], m.h, m.n[2]);
-  const int32_t tc = m.tcell_base + (tx * m.n[1] + ty) * m.n[2] + tz;
-  const int32_t k0 = tcell_off[tc], k1 = tcell_off[tc + 1];
-  if (k0 == k1) return;
-  const T r = x.w;
-  const Vec3<T> p = {x.x, x.y, x.z};
-  const T lox = x.x - r, loy = x.y - r, loz = x.z - r;
-  const T hix = x.x + r, hiy = x.y + r, hiz = x.z + r;
-  const T e = s->ew;
-  Vec3<T> y = p;
-  T ax = T(0), ay = T(0), az = T(0);
-  int32_t cnt = 0;
-  for (int32_t k = k0; k < k1; ++k) {
-    const TriDev<T> t = tris[tcell_tri[k]];
-    if (lox > t.hi[0] || hix < t.lo[0] || loy > t.hi[1] || hiy < t.lo[1] || loz > t.hi[2] ||
-        hiz < t.lo[2])
-      continue;
-    const Vec3<T> q = closest_on_triangle(t, p);
-    if (dist_to(p, q) < r) {
-      const T s2 = T(2) * dot3(v.x, v.y, v.z, t.n[0], t.n[1], t.n[2]);
-      ax = ax + e * (v.x - s2 * t.n[0]);
-      ay = ay + e * (v.y - s2 * t.n[1]);
-      az = az + e * (v.z - s2 * t.n[2]);
-      const T dtp = dot3(-t.n[0], -t.n[1], -t.n[2], y.x - t.a[0], y.y - t.a[1], y.z - t.a[2]);
-      const T mod = r - dtp;
-      y.x = y.x - mod * t.n[0];
-      y.y = y.y - mod * t.n[1];
-      y.z = y.z - mod * t.n[2];
-      ++cnt;
-    }
+struct MeshAcc {
+  Vec3<T> y;
+  T ax, ay, az;
+  int32_t cnt;
+};
+
+// One hit: acc += e * (v - 2 (v.n) n); y -= (r - dot(-n, y - a)) n (PolygonContainer.cpp:195-211).
+template <class T>
+__device__ __forceinline__ void mesh_hit(MeshAcc<T>& m, T e, T r, T vx, T vy, T vz, T nx, T ny,
+                                         T nz, T a0, T a1, T a2) {
+  const T s2 = T(2) * dot3(vx, vy, vz, nx, ny, nz);
+  m.ax = m.ax + e * (vx - s2 * nx);
+  m.ay = m.ay + e * (vy - s2 * ny);
+  m.az = m.az + e * (vz - s2 * nz);
+  const T dtp = dot3(-nx, -ny, -nz, m.y.x - a0, m.y.y - a1, m.y.z - a2);
+  const T mod = r - dtp;
+  m.y.x = m.y.x - mod * nx;
+  m.y.y = m.y.y - mod * ny;
+  m.y.z = m.y.z - mod * nz;
+  ++m.cnt;
+}
+
+// AABB reject (PolygonContainer.cpp:251-259), closest point and contact test of one candidate.
+template <class T>
+__device__ __forceinline__ bool mesh_touch(const TriBox<T>& bx, const TriGeo<T>* __restrict__ tgeo,
+                                           const Vec3<T>& P, T r, TriGeo<T>& t) {
+  if (P.x - r > bx.hi[0] || P.x + r < bx.lo[0] || P.y - r > bx.hi[1] || P.y + r < bx.lo[1] ||
+      P.z - r > bx.hi[2] || P.z + r < bx.lo[2])
+    return false;
+  t = tgeo[bx.tri];
+  const Vec3<T> q = closest_on_triangle(t, P);
+  return dist_to(P, q) < r;
+}
+
+template <class T>
+__device__ inline void mesh_finish(const MeshAcc<T>& m, Vec4<T>& x, Vec4<T>& v) {
+  if (m.cnt > 0) {  // PolygonContainer.cpp:280-285
+    const T cf = (T)m.cnt;
+    x.x = m.y.x;
+    x.y = m.y.y;
+    x.z = m.y.z;
+    v.x = m.ax / cf;
+    v.y = m.ay / cf;
+    v.z = m.az / cf;
   }
-  if (cnt > 0) {
-    const T cf = (T)cnt;
-    x.x = y.x;
-    x.y = y.y;
-    x.z = y.z;
-    v.x = ax / cf;
-    v.y = ay / cf;
-    v.z = az / cf;
+}
+
+// Lists longer than this are walked by the whole wave: 64 lanes test 64 candidates at once and the
+// hits are then applied in ascending order, one after the other, exactly as a single lane would.
+constexpr int32_t kHeavyList = 12;
+
+template <class T>
+__device__ inline void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool active, int32_t c,
+                                         const TriGeo<T>* __restrict__ tgeo,
+                                         const TriBox<T>* __restrict__ tbox,
+                                         const int32_t* __restrict__ tcell_off, Vec4<T>& x,
+                                         Vec4<T>& v) {
+  int32_t k0 = 0, k1 = 0;
+  if (active) {
+    const int32_t tc = s->tcell_base + (c - s->grid.cell_base);
+    k0 = tcell_off[tc];
+    k1 = tcell_off[tc + 1];
+  }
+  const T e = s->ew;
+  const bool heavy = (k1 - k0) > kHeavyList;
+  if (active && !heavy && k1 > k0) {
+    const Vec3<T> P = {x.x, x.y, x.z};
+    MeshAcc<T> m = {P, T(0), T(0), T(0), 0};
+    for (int32_t k = k0; k < k1; ++k) {
+      TriGeo<T> t;
+      if (mesh_touch(tbox[k], tgeo, P, x.w, t))
+        mesh_hit(m, e, x.w, v.x, v.y, v.z, t.n[0], t.n[1], t.n[2], t.a[0], t.a[1], t.a[2]);
+    }
+    mesh_finish(m, x, v);
+  }
+  uint64_t hm = __ballot(active && heavy);
+  while (hm) {
+    const int owner = __ffsll((unsigned long long)hm) - 1;
+    hm &= hm - 1;
+    const Vec3<T> P = {__shfl(x.x, owner), __shfl(x.y, owner), __shfl(x.z, owner)};
+    const T r = __shfl(x.w, owner);
+    const T vx = __shfl(v.x, owner), vy = __shfl(v.y, owner), vz = __shfl(v.z, owner);
+    const int32_t b0 = __shfl(k0, owner), b1 = __shfl(k1, owner);
+    MeshAcc<T> m = {P, T(0), T(0), T(0), 0};
+    for (int32_t base = b0; base < b1; base += 64) {
+      const int32_t k = base + lane_id();
+      TriGeo<T> t = {};
+      const bool hit = (k < b1) && mesh_touch(tbox[k], tgeo, P, r, t);
+      uint64_t hits = __ballot(hit);
+      while (hits) {  // ascending lane = ascending candidate
+        const int src = __ffsll((unsigned long long)hits) - 1;
+        hits &= hits - 1;
+        mesh_hit(m, e, r, vx, vy, vz, __shfl(t.n[0], src), __shfl(t.n[1], src),
+                 __shfl(t.n[2], src), __shfl(t.a[0], src), __shfl(t.a[1], src),
+                 __shfl(t.a[2], src));
+      }
+    }
+    if (lane_id() == owner) mesh_finish(m, x, v);
   }
 }
 
 // ---------------------------------------------------------------------------------------------
-// K1: integrate + cell hash + container + count
+// K1: integrate + cell hash + container + counts
 // ---------------------------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(1024)
-    k_integrate(const BlockInfo* __restrict__ btab, const SceneDev<T>* __restrict__ scenes,
-                Vec4<T>* __restrict__ pos, Vec4<T>* __restrict__ vel, int32_t* __restrict__ cell_of,
+    k_integrate(int32_t ntab, const BlockInfo* __restrict__ btab,
+                const SceneDev<T>* __restrict__ scenes, Vec4<T>* __restrict__ pos,
+                Vec4<T>* __restrict__ vel, int32_t* __restrict__ cell_of,
                 uint32_t* __restrict__ slot, uint32_t* __restrict__ count,
-                const MeshDev* __restrict__ meshes, const TriDev<T>* __restrict__ tris,
-                const int32_t* __restrict__ tcell_off, const int32_t* __restrict__ tcell_tri,
+                const TriGeo<T>* __restrict__ tgeo,
+                const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
                 uint32_t* __restrict__ flags, int mode) {
-  const BlockInfo bi = btab[blockIdx.x];
+  const int32_t tb = xcd_tile(blockIdx.x, ntab);
+  if (tb >= ntab) return;
+  const BlockInfo bi = btab[tb];
   const int32_t p = bi.p_start + (int32_t)threadIdx.x;
-  if (p >= bi.p_end) return;
+  const bool active = p < bi.p_end;
   const SceneDev<T>* __restrict__ s = scenes + bi.scene;
-  Vec4<T> x = pos[p];
-  Vec4<T> v = vel[p];
-  if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25
-    const T dt = s->dt;
-    x.x = x.x + v.x * dt;
-    x.y = x.y + v.y * dt;
-    x.z = x.z + v.z * dt;
-    v.x = v.x + s->gdt[0];
-    v.y = v.y + s->gdt[1];
-    v.z = v.z + s->gdt[2];
+  Vec4<T> x = {}, v = {};
+  int32_t c = 0;
+  if (active) {
+    x = pos[p];
+    v = vel[p];
+    if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25
+      const T dt = s->dt;
+      x.x = x.x + v.x * dt;
+      x.y = x.y + v.y * dt;
+      x.z = x.z + v.z * dt;
+      v.x = v.x + s->gdt[0];
+      v.y = v.y + s->gdt[1];
+      v.z = v.z + s->gdt[2];
+    }
+    // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container x.
+    const GridDev& g = s->grid;
+    const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+    const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+    const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+    c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
   }
-  // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container position.
-  const GridDev& g = s->grid;
-  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
   if (mode & kModeContainer) {
-    if (s->mesh < 0)
-      box_walls(s, x, v);
-    else
-      mesh_contact(s, meshes, tris, tcell_off, tcell_tri, x, v);
+    if (s->mesh < 0) {
+      if (active) box_walls(s, x, v);
+    } else {
+      mesh_contact_wave(s, active, c, tgeo, tbox, tcell_off, x, v);
+    }
   }
-  if (mode & kModeCheck) {
-    const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
-                    isfinite(v.y) && isfinite(v.z);
-    if (!ok) atomicOr(flags, kFlagNonFinite);
+  if (active) {
+    if (mode & kModeCheck) {
+      const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
+                      isfinite(v.y) && isfinite(v.z);
+      if (!ok) atomicOr(flags, kFlagNonFinite);
+    }
+    if (mode & (kModeIntegrate | kModeContainer)) {
+      pos[p] = x;
+      vel[p] = v;
+    }
+    cell_of[p] = c;
+    slot[p] = atomicAdd(count + c, 1u);
   }
-  if (mode & (kModeIntegrate | kModeContainer)) {
-    pos[p] = x;
-    vel[p] = v;
-  }
-  cell_of[p] = c;
-  slot[p] = atomicAdd(count + c, 1u);
 }
 
 // ---------------------------------------------------------------------------------------------
-// K3: exclusive scan of the per-cell counts (tile = 256 threads x 16 cells). Reduce, scan of the
-// tile sums in one workgroup, then a down-sweep that rescans each tile with wave64 inclusive scans
-// and LDS wave totals, adds the tile prefix, writes the offsets and clears the counts.
+// K3: exclusive scan of the per-cell counts over tiles of kScanTile cells: the tile sums, one
+// workgroup turning them into tile prefixes, then every tile rescanned (16 cells per thread summed
+// serially, wave64 inclusive scans of the thread totals, LDS wave totals, the tile prefix added);
+// offsets written, counts cleared for the next step.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kScanThreads)
     k_scan_reduce(const uint32_t* __restrict__ count, uint32_t* __restrict__ tile_sums) {
@@ -190,12 +267,13 @@ __global__ void __launch_bounds__(kScanThreads)
       reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile);
   uint32_t s = 0;
 #pragma unroll
-  for (int q = 0; q < kScanPerThread / 4; ++q) {
+  for (int q = 0; q < kScanPerThread / 4; ++q) {  // coalesced: consecutive lanes, consecutive uint4
     const uint4 u = src[q * kScanThreads + threadIdx.x];
     s += (u.x + u.y) + (u.z + u.w);
   }
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
@@ -205,7 +283,8 @@ __global__ void __launch_bounds__(kScanThreads)
   }
 }
 
-__global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* __restrict__ tile_sums, int32_t ntiles) {
+__global__ void __launch_bounds__(1024)
+    k_scan_tiles(uint32_t* __restrict__ tile_sums, uint32_t* __restrict__ tile_pre, int32_t ntiles) {
   __shared__ uint32_t wsum[16];
   const int32_t per = (ntiles + 1023) / 1024;
   const int32_t b = (int32_t)threadIdx.x * per;
@@ -213,23 +292,22 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* __restrict__ tile
   uint32_t s = 0;
   for (int32_t i = b; i < e; ++i) s += tile_sums[i];
   const uint32_t wi = wave_inclusive_scan(s);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 63) wsum[w] = wi;
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 63) wsum[w] = wi;
   __syncthreads();
-  uint32_t wpre = 0;
-  for (int q = 0; q < w; ++q) wpre += wsum[q];
-  uint32_t run = wpre + wi - s;
+  uint32_t run = wi - s;
+  for (int q = 0; q < w; ++q) run += wsum[q];
   for (int32_t i = b; i < e; ++i) {
     const uint32_t c = tile_sums[i];
-    tile_sums[i] = run;
+    tile_pre[i] = run;
     run += c;
   }
-  if (threadIdx.x == 1023) tile_sums[ntiles] = run;
+  if (threadIdx.x == 1023) tile_pre[ntiles] = run;
 }
 
 __global__ void __launch_bounds__(kScanThreads)
     k_scan_down(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
-                const uint32_t* __restrict__ tile_sums, int32_t ntiles) {
+                const uint32_t* __restrict__ tile_pre, int32_t ntiles) {
   __shared__ uint32_t wsum[kScanThreads / 64];
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPerThread;
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + base);
@@ -246,10 +324,10 @@ __global__ void __launch_bounds__(kScanThreads)
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) tot += v[k];
   const uint32_t wi = wave_inclusive_scan(tot);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 63) wsum[w] = wi;
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 63) wsum[w] = wi;
   __syncthreads();
-  uint32_t run = tile_sums[blockIdx.x] + wi - tot;
+  uint32_t run = tile_pre[blockIdx.x] + wi - tot;
   for (int q = 0; q < w; ++q) run += wsum[q];
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
@@ -266,17 +344,17 @@ __global__ void __launch_bounds__(kScanThreads)
     reinterpret_cast<uint4*>(offset + base)[q] = o;
     src[q] = zero;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) offset[(size_t)ntiles * kScanTile] = tile_sums[ntiles];
+  if (blockIdx.x == 0 && threadIdx.x == 0) offset[(size_t)ntiles * kScanTile] = tile_pre[ntiles];
 }
 
 // ---------------------------------------------------------------------------------------------
 // K4: unordered cell lists, then stable rank + gather
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024)
-    k_scatter_ids(int32_t n, const int32_t* __restrict__ cell_of, const uint32_t* __restrict__ slot,
-                  const uint32_t* __restrict__ offset, const int32_t* __restrict__ id,
-                  int32_t* __restrict__ tmp) {
-  const int32_t p = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    k_scatter_ids(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
+                  const uint32_t* __restrict__ slot, const uint32_t* __restrict__ offset,
+                  const int32_t* __restrict__ id, int32_t* __restrict__ tmp) {
+  const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
   if (p >= n) return;
   tmp[offset[cell_of[p]] + slot[p]] = id[p];
 }
@@ -285,12 +363,12 @@ __global__ void __launch_bounds__(1024)
 // ParticleModel.cpp:55-61): rank = number of cell mates with a smaller id.
 template <class T>
 __global__ void __launch_bounds__(1024)
-    k_rank_gather(int32_t n, const int32_t* __restrict__ cell_of, const uint32_t* __restrict__ offset,
-                  const int32_t* __restrict__ tmp, const Vec4<T>* __restrict__ pos,
-                  const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
-                  Vec4<T>* __restrict__ pos_o, Vec4<T>* __restrict__ vel_o,
-                  int32_t* __restrict__ id_o) {
-  const int32_t p = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    k_rank_gather(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
+                  const uint32_t* __restrict__ offset, const int32_t* __restrict__ tmp,
+                  const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
+                  const int32_t* __restrict__ id, Vec4<T>* __restrict__ pos_o,
+                  Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o) {
+  const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
   if (p >= n) return;
   const int32_t c = cell_of[p];
   const uint32_t b = offset[c], e = offset[c + 1];
@@ -310,56 +388,97 @@ __global__ void __launch_bounds__(1024)
 // column are one contiguous slice [offset[c(zlo)], offset[c(zhi)+1]) of the sorted arrays, and
 // walking it visits cells in order and members by ascending id -- the reference's summation order.
 // ---------------------------------------------------------------------------------------------
+
+// Per-lane constants of particle i.
+template <class T>
+struct PairSelf {
+  Vec4<T> x, v;
+  T thr;  // (r_i + r_max)^2 (1 + margin): no candidate beyond it can be in contact
+  T ks, es;
+};
+
+// One candidate j of particle i. The prefilter (fused multiply-adds, conservative by the margin)
+// only skips candidates the exact test would reject; the test and the kick are evaluated in the
+// reference's expression order.
+template <class T>
+__device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4<T>& xj,
+                                               const Vec4<T>* __restrict__ vsn, uint32_t q,
+                                               T& ax, T& ay, T& az) {
+  // isHit: distance(x_i, x_j) = length(x_j - x_i) < r_i + r_j
+  const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
+  if (fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr) return;
+  const T dd = dot3(ex, ey, ez, ex, ey, ez);
+  const T R = me.x.w + xj.w;
+  if (dd > (R * R) * PairMargin<T>::value) return;
+  const T len = sqrt(dd);
+  if (!(len < R)) return;
+  const Vec4<T> vj = vsn[q];
+  // d = x_i - x_j; dot(d, d) and length(d) equal dd and len bit for bit.
+  const T dx = -ex, dy = -ey, dz = -ez;
+  const T ov = R - len;
+  const T s3 = ((T(2) * vj.w) / (me.v.w + vj.w)) *
+               (dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz) / dd);
+  const T inv = T(1) / len;
+  const T kov = me.ks * ov;
+  ax = ax + ((-((s3 * dx) * me.es)) + kov * (dx * inv));
+  ay = ay + ((-((s3 * dy) * me.es)) + kov * (dy * inv));
+  az = az + ((-((s3 * dz) * me.es)) + kov * (dz * inv));
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024)
-    k_pairs(const BlockInfo* __restrict__ btab, const SceneDev<T>* __restrict__ scenes,
-            const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vsn,
-            const uint32_t* __restrict__ offset, Vec4<T>* __restrict__ vout) {
-  const BlockInfo bi = btab[blockIdx.x];
+    k_pairs(int32_t ntab, const BlockInfo* __restrict__ btab,
+            const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ pos,
+            const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
+            Vec4<T>* __restrict__ vout) {
+  const int32_t tb = xcd_tile(blockIdx.x, ntab);
+  if (tb >= ntab) return;
+  const BlockInfo bi = btab[tb];
   const int32_t p = bi.p_start + (int32_t)threadIdx.x;
   if (p >= bi.p_end) return;
   const SceneDev<T>* __restrict__ s = scenes + bi.scene;
   const GridDev& g = s->grid;
-  const Vec4<T> xi = pos[p];
-  const Vec4<T> vi = vsn[p];
-  const int32_t hx = axis_index((double)xi.x, g.min[0], g.h, g.n[0]);
-  const int32_t hy = axis_index((double)xi.y, g.min[1], g.h, g.n[1]);
-  const int32_t hz = axis_index((double)xi.z, g.min[2], g.h, g.n[2]);
+  PairSelf<T> me;
+  me.x = pos[p];
+  me.v = vsn[p];
+  {
+    const T rr = me.x.w + s->rmax;
+    me.thr = (rr * rr) * PairMargin<T>::value;
+    me.ks = s->spring;
+    me.es = s->es;
+  }
+  // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped)
+  const int32_t hx = axis_index((double)me.x.x, g.min[0], g.h, g.n[0]);
+  const int32_t hy = axis_index((double)me.x.y, g.min[1], g.h, g.n[1]);
+  const int32_t hz = axis_index((double)me.x.z, g.min[2], g.h, g.n[2]);
   const int32_t zlo = max(hz - 1, 0), zhi = min(hz + 1, g.n[2] - 1);
-  const T ri = xi.w, mi = vi.w, ks = s->spring, es = s->es;
-  const T margin = PairMargin<T>::value;
-  T ax = vi.x, ay = vi.y, az = vi.z;
-  for (int32_t bx = hx - 1; bx <= hx + 1; ++bx) {
-    if (bx < 0 || bx >= g.n[0]) continue;
-    for (int32_t by = hy - 1; by <= hy + 1; ++by) {
-      if (by < 0 || by >= g.n[1]) continue;
+  uint32_t lo[9], hi[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const int32_t bx = hx - 1 + d / 3, by = hy - 1 + d % 3;
+    lo[d] = 0;
+    hi[d] = 0;
+    if (bx >= 0 && bx < g.n[0] && by >= 0 && by < g.n[1]) {
       const int32_t cb = g.cell_base + (bx * g.n[1] + by) * g.n[2];
-      const uint32_t q0 = offset[cb + zlo], q1 = offset[cb + zhi + 1];
-      for (uint32_t q = q0; q < q1; ++q) {
-        if ((int32_t)q == p) continue;
-        const Vec4<T> xj = pos[q];
-        // isHit: distance(x_i, x_j) = length(x_j - x_i) < r_i + r_j
-        const T ex = xj.x - xi.x, ey = xj.y - xi.y, ez = xj.z - xi.z;
-        const T dd = dot3(ex, ey, ez, ex, ey, ez);
-        const T R = ri + xj.w;
-        if (dd > (R * R) * margin) continue;
-        const T len = sqrt(dd);
-        if (!(len < R)) continue;
-        const Vec4<T> vj = vsn[q];
-        // d = x_i - x_j; dot(d, d) and length(d) equal dd and len bit for bit.
-        const T dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const T ov = R - len;
-        const T s3 = ((T(2) * vj.w) / (mi + vj.w)) *
-                     (dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, dx, dy, dz) / dd);
-        const T inv = T(1) / len;
-        const T kov = ks * ov;
-        ax = ax + ((-((s3 * dx) * es)) + kov * (dx * inv));
-        ay = ay + ((-((s3 * dy) * es)) + kov * (dy * inv));
-        az = az + ((-((s3 * dz) * es)) + kov * (dz * inv));
+      lo[d] = offset[cb + zlo];
+      hi[d] = offset[cb + zhi + 1];
+    }
+  }
+  T ax = me.v.x, ay = me.v.y, az = me.v.z;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    uint32_t q = lo[d];
+    const uint32_t e = hi[d];
+    if (q < e) {
+      Vec4<T> nxt = pos[q];  // one candidate in flight ahead of the one being tested
+      for (; q < e; ++q) {
+        const Vec4<T> cur = nxt;
+        if (q + 1 < e) nxt = pos[q + 1];
+        if ((int32_t)q != p) pair_candidate(me, cur, vsn, q, ax, ay, az);
       }
     }
   }
-  vout[p] = Vec4<T>{ax, ay, az, mi};
+  vout[p] = Vec4<T>{ax, ay, az, me.v.w};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -413,17 +532,16 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline int32_t blocks_for(int64_t n, int b) { return (int32_t)((n + b - 1) / b); }
 
 template <class T>
-hipError_t launch_integrate(hipStream_t st, int nblocks, int block, const BlockInfo* btab,
+hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, Vec4<T>* pos, Vec4<T>* vel,
                             int32_t* cell_of, uint32_t* slot, uint32_t* count,
-                            const MeshDev* meshes, const TriDev<T>* tris,
-                            const int32_t* tcell_off, const int32_t* tcell_tri, uint32_t* flags,
-                            int mode) {
-  hipLaunchKernelGGL(k_integrate<T>, dim3(nblocks), dim3(block), 0, st, btab, scenes, pos, vel,
-                     cell_of, slot, count, meshes, tris, tcell_off, tcell_tri, flags, mode);
+                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
+                            const int32_t* tcell_off, uint32_t* flags, int mode) {
+  hipLaunchKernelGGL(k_integrate<T>, dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
+                     pos, vel, cell_of, slot, count, tgeo, tbox, tcell_off, flags, mode);
   return hipGetLastError();
 }
 
@@ -433,22 +551,24 @@ hipError_t launch_scan_reduce(hipStream_t st, const uint32_t* count, uint32_t* t
   return hipGetLastError();
 }
 
-hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, int32_t ntiles) {
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, ntiles);
+hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, uint32_t* tile_pre,
+                             int32_t ntiles) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, tile_pre, ntiles);
   return hipGetLastError();
 }
 
 hipError_t launch_scan_down(hipStream_t st, uint32_t* count, uint32_t* offset,
-                            const uint32_t* tile_sums, int32_t ntiles) {
-  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset,
-                     tile_sums, ntiles);
+                            const uint32_t* tile_pre, int32_t ntiles) {
+  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_pre,
+                     ntiles);
   return hipGetLastError();
 }
 
 hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
                               const uint32_t* slot, const uint32_t* offset, const int32_t* id,
                               int32_t* tmp) {
-  hipLaunchKernelGGL(k_scatter_ids, dim3(blocks_for(n, block)), dim3(block), 0, st, n, cell_of,
+  const int32_t tiles = blocks_for(n, block);
+  hipLaunchKernelGGL(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
                      slot, offset, id, tmp);
   return hipGetLastError();
 }
@@ -458,17 +578,18 @@ hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_
                               const uint32_t* offset, const int32_t* tmp, const Vec4<T>* pos,
                               const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
                               Vec4<T>* vel_o, int32_t* id_o) {
-  hipLaunchKernelGGL(k_rank_gather<T>, dim3(blocks_for(n, block)), dim3(block), 0, st, n, cell_of,
-                     offset, tmp, pos, vel, id, pos_o, vel_o, id_o);
+  const int32_t tiles = blocks_for(n, block);
+  hipLaunchKernelGGL(k_rank_gather<T>, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles,
+                     cell_of, offset, tmp, pos, vel, id, pos_o, vel_o, id_o);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int nblocks, int block, const BlockInfo* btab,
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
                         const uint32_t* offset, Vec4<T>* vout) {
-  hipLaunchKernelGGL(k_pairs<T>, dim3(nblocks), dim3(block), 0, st, btab, scenes, pos, vsn, offset,
-                     vout);
+  hipLaunchKernelGGL(k_pairs<T>, dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes, pos,
+                     vsn, offset, vout);
   return hipGetLastError();
 }
 
@@ -488,21 +609,21 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
   return hipGetLastError();
 }
 
-#define PSX_INSTANTIATE(T)                                                                       \
-  template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,              \
-                                          const SceneDev<T>*, Vec4<T>*, Vec4<T>*, int32_t*,     \
-                                          uint32_t*, uint32_t*, const MeshDev*, const TriDev<T>*, \
-                                          const int32_t*, const int32_t*, uint32_t*, int);      \
-  template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,           \
-                                            const uint32_t*, const int32_t*, const Vec4<T>*,    \
-                                            const Vec4<T>*, const int32_t*, Vec4<T>*, Vec4<T>*, \
-                                            int32_t*);                                          \
-  template hipError_t launch_pairs<T>(hipStream_t, int, int, const BlockInfo*,                  \
-                                      const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,       \
-                                      const uint32_t*, Vec4<T>*);                               \
-  template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,     \
-                                       const int32_t*, T*, T*);                                 \
-  template hipError_t launch_sort_in<T>(hipStream_t, int32_t, Vec4<T>*, Vec4<T>*,                \
+#define PSX_INSTANTIATE(T)                                                                      \
+  template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
+                                          const SceneDev<T>*, Vec4<T>*, Vec4<T>*, int32_t*,    \
+                                          uint32_t*, uint32_t*, const TriGeo<T>*,              \
+                                          const TriBox<T>*, const int32_t*, uint32_t*, int);   \
+  template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,          \
+                                            const uint32_t*, const int32_t*, const Vec4<T>*,   \
+                                            const Vec4<T>*, const int32_t*, Vec4<T>*, Vec4<T>*,\
+                                            int32_t*);                                         \
+  template hipError_t launch_pairs<T>(hipStream_t, int, int, const BlockInfo*,                 \
+                                      const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
+                                      const uint32_t*, Vec4<T>*);                              \
+  template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,    \
+                                       const int32_t*, T*, T*);                                \
+  template hipError_t launch_sort_in<T>(hipStream_t, int32_t, Vec4<T>*, Vec4<T>*,               \
                                         const int32_t*, const T*, const T*);
 
 PSX_INSTANTIATE(float)

@@ -13,18 +13,18 @@ constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 
 template <class T>
-hipError_t launch_integrate(hipStream_t st, int nblocks, int block, const BlockInfo* btab,
+hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, Vec4<T>* pos, Vec4<T>* vel,
                             int32_t* cell_of, uint32_t* slot, uint32_t* count,
-                            const MeshDev* meshes, const TriDev<T>* tris,
-                            const int32_t* tcell_off, const int32_t* tcell_tri, uint32_t* flags,
-                            int mode);
+                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
+                            const int32_t* tcell_off, uint32_t* flags, int mode);
 
 hipError_t launch_scan_reduce(hipStream_t st, const uint32_t* count, uint32_t* tile_sums,
                               int32_t ntiles);
-hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, int32_t ntiles);
+hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, uint32_t* tile_pre,
+                             int32_t ntiles);
 hipError_t launch_scan_down(hipStream_t st, uint32_t* count, uint32_t* offset,
-                            const uint32_t* tile_sums, int32_t ntiles);
+                            const uint32_t* tile_pre, int32_t ntiles);
 
 hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
                               const uint32_t* slot, const uint32_t* offset, const int32_t* id,
@@ -37,7 +37,7 @@ hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_
                               Vec4<T>* vel_o, int32_t* id_o);
 
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int nblocks, int block, const BlockInfo* btab,
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
                         const uint32_t* offset, Vec4<T>* vout);
 

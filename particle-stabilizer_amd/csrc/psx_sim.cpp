@@ -137,73 +137,50 @@ double point_triangle_distance(D3 p, D3 a, D3 b, D3 c) {
 
 }  // namespace
 
-// Builds the triangle grid of one mesh. A cell lists triangle t (ascending) when a sphere of radius
-// <= rmax whose centre hashes to that cell can be within contact distance of t:
-//   interior cells: distance(cell centre, t) <= half diagonal + rmax + margin;
-//   boundary cells (the hash clamps positions outside the domain into them, so they are unbounded
-//   on their outer side): t's AABB overlaps the cell box extended to infinity outward, grown by
-//   rmax + margin.
-// margin absorbs the rounding of the hash division and of the closest-point evaluation in T.
-void build_tri_grid(MeshHost& m, double cell, double rmax, bool fp32) {
+// Builds the triangle lists of one mesh over the scene's particle grid g (whose domain is the mesh's
+// bounding box, PhysicsEngine.cpp:112). Cell k lists triangle t (ascending) when a sphere of radius
+// <= rmax hashed to cell k can touch t. The hash clamps positions outside the domain into the
+// boundary cells, but a sphere farther than rmax from the domain cannot touch any triangle, so the
+// boundary cells' boxes are extended outward by rmax + margin only. A cell keeps t when
+//   distance(box centre, t) <= half diagonal of the box + rmax + margin,
+// a superset of the triangles within rmax of the box. margin absorbs the rounding of the hash
+// division and of the closest-point evaluation in T.
+void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32) {
   const int64_t nt = m.nt;
-  for (int a = 0; a < 3; ++a) {
-    const double ext = m.max[a] - m.min[a];
-    m.tn[a] = (int32_t)std::max<int64_t>(1, (int64_t)(ext / cell) + 1);
-  }
-  // keep the table modest: coarsen until it has at most 8M cells
-  while ((int64_t)m.tn[0] * m.tn[1] * m.tn[2] > (int64_t)8 << 20) {
-    cell *= 1.25;
-    for (int a = 0; a < 3; ++a)
-      m.tn[a] = (int32_t)std::max<int64_t>(1, (int64_t)((m.max[a] - m.min[a]) / cell) + 1);
-  }
-  m.th = cell;
   double lmax = 0.0;
   for (int a = 0; a < 3; ++a) lmax = std::max(lmax, std::max(std::fabs(m.min[a]), std::fabs(m.max[a])));
   const double margin = fp32 ? 1e-4 * (lmax + 1.0) + 1e-3 * rmax : 1e-9 * (lmax + 1.0);
-  const double half_diag = 0.5 * std::sqrt(3.0) * cell;
-  const double reach_in = half_diag + rmax + margin;  // interior sphere test radius
-  const double grow = rmax + margin;                  // boundary AABB test growth
-  const int64_t ncell = (int64_t)m.tn[0] * m.tn[1] * m.tn[2];
+  const double reach = rmax + margin;
+  const int64_t ncell = g.n[0] * g.n[1] * g.n[2];
   std::vector<std::vector<int32_t>> lists((size_t)ncell);
-  auto idx = [&](double x, int a) {
-    return (int64_t)axis_index(x, m.min[a], m.th, m.tn[a]);
+  auto box_lo = [&](int64_t i, int a) {
+    return i == 0 ? g.min[a] - reach : g.min[a] + (double)i * g.h;
+  };
+  auto box_hi = [&](int64_t i, int a) {
+    return i == g.n[a] - 1 ? g.min[a] + (double)g.n[a] * g.h + reach : g.min[a] + (double)(i + 1) * g.h;
   };
   for (int64_t t = 0; t < nt; ++t) {
     const double* p = &m.v[9 * t];
     const D3 A{p[0], p[1], p[2]}, B{p[3], p[4], p[5]}, C{p[6], p[7], p[8]};
-    double lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = std::min(p[a], std::min(p[3 + a], p[6 + a]));
-      hi[a] = std::max(p[a], std::max(p[3 + a], p[6 + a]));
-    }
-    const double reach = std::max(reach_in, grow + cell);
     int64_t i0[3], i1[3];
     for (int a = 0; a < 3; ++a) {
-      i0[a] = idx(lo[a] - reach, a);
-      i1[a] = idx(hi[a] + reach, a);
+      const double lo = std::min(p[a], std::min(p[3 + a], p[6 + a]));
+      const double hi = std::max(p[a], std::max(p[3 + a], p[6 + a]));
+      i0[a] = axis_index(lo - reach, g.min[a], g.h, (int32_t)g.n[a]);
+      i1[a] = axis_index(hi + reach, g.min[a], g.h, (int32_t)g.n[a]);
     }
     for (int64_t ix = i0[0]; ix <= i1[0]; ++ix)
       for (int64_t iy = i0[1]; iy <= i1[1]; ++iy)
         for (int64_t iz = i0[2]; iz <= i1[2]; ++iz) {
           const int64_t ii[3] = {ix, iy, iz};
-          bool boundary = false;
-          for (int a = 0; a < 3; ++a) boundary |= (ii[a] == 0 || ii[a] == m.tn[a] - 1);
-          bool take;
-          if (boundary) {
-            take = true;
-            for (int a = 0; a < 3; ++a) {
-              const double clo = ii[a] == 0 ? -std::numeric_limits<double>::infinity()
-                                            : m.min[a] + (double)ii[a] * m.th;
-              const double chi = ii[a] == m.tn[a] - 1 ? std::numeric_limits<double>::infinity()
-                                                      : m.min[a] + (double)(ii[a] + 1) * m.th;
-              if (lo[a] > chi + grow || hi[a] < clo - grow) take = false;
-            }
-          } else {
-            const D3 ctr{m.min[0] + ((double)ix + 0.5) * m.th, m.min[1] + ((double)iy + 0.5) * m.th,
-                         m.min[2] + ((double)iz + 0.5) * m.th};
-            take = point_triangle_distance(ctr, A, B, C) <= reach_in;
+          double ctr[3], hd2 = 0.0;
+          for (int a = 0; a < 3; ++a) {
+            const double lo = box_lo(ii[a], a), hi = box_hi(ii[a], a);
+            ctr[a] = 0.5 * (lo + hi);
+            hd2 += 0.25 * (hi - lo) * (hi - lo);
           }
-          if (take) lists[(size_t)((ix * m.tn[1] + iy) * m.tn[2] + iz)].push_back((int32_t)t);
+          if (point_triangle_distance(D3{ctr[0], ctr[1], ctr[2]}, A, B, C) <= std::sqrt(hd2) + reach)
+            lists[(size_t)((ix * g.n[1] + iy) * g.n[2] + iz)].push_back((int32_t)t);
         }
   }
   m.off.assign((size_t)ncell + 1, 0);
@@ -260,8 +237,8 @@ struct SimImpl final : SimBase {
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
   DevBuf cell_of, slot, tmp;
-  DevBuf count, offset, tile_sums, flags;
-  DevBuf scenes_d, meshes_d, tris_d, tcell_off_d, tcell_tri_d;
+  DevBuf count, offset, tile_sums, tile_pre, flags;
+  DevBuf scenes_d, tgeo_d, tbox_d, tcell_off_d;
   DevBuf io;  // 6n T staging for state transfer
   std::map<int, std::unique_ptr<DevBuf>> btab;
   std::map<int, int> btab_blocks;
@@ -318,9 +295,9 @@ struct SimImpl final : SimBase {
     // scenes: r_max in T, grid, cell bases (ParticleModel::initBuckets, ParticleModel.cpp:28-47)
     int64_t cells = 0;
     std::vector<SceneDev<T>> sd(scenes.size());
-    std::vector<MeshDev> md;
-    std::vector<TriDev<T>> tris;
-    std::vector<int32_t> toff, ttri;
+    std::vector<TriGeo<T>> tgeo;
+    std::vector<TriBox<T>> tbox;
+    std::vector<int32_t> toff;
     for (size_t s = 0; s < scenes.size(); ++s) {
       SceneHost& sh = scenes[s];
       double rmax = 0.0;
@@ -375,51 +352,50 @@ struct SimImpl final : SimBase {
       d.p_begin = (int32_t)sh.p_begin;
       d.p_end = (int32_t)(sh.p_begin + sh.n);
       d.mesh = -1;
+      d.tcell_base = 0;
       if (sh.mesh >= 0) {
         MeshHost& m = meshes[(size_t)sh.mesh];
-        build_tri_grid(m, 2.0 * sh.grid.h, rmax, sizeof(T) == 4);
-        MeshDev mdv;
-        std::memset(&mdv, 0, sizeof(mdv));
-        for (int a = 0; a < 3; ++a) {
-          mdv.min[a] = m.min[a];
-          mdv.n[a] = m.tn[a];
-        }
-        mdv.h = m.th;
-        mdv.tcell_base = (int32_t)toff.size();
-        const int32_t tri_base = (int32_t)tris.size();
-        for (size_t c2 = 0; c2 + 1 < m.off.size(); ++c2) toff.push_back((int32_t)ttri.size() + m.off[c2]);
-        for (int32_t t : m.list) ttri.push_back(tri_base + t);
-        // triangle records (PolygonContainer.cpp:52-89): normal in double, stored in T
+        build_tri_grid(m, sh.grid, rmax, sizeof(T) == 4);
+        d.mesh = sh.mesh;
+        d.tcell_base = (int32_t)toff.size();
+        const int32_t tri_base = (int32_t)tgeo.size();
+        // triangle records (PolygonContainer.cpp:52-89): normal in double, stored in T; AABB over
+        // the T corners (glm::min / glm::max)
+        std::vector<TriBox<T>> boxes((size_t)m.nt);
         for (int64_t t = 0; t < m.nt; ++t) {
-          const double* q = &m.v[9 * t];
+          const double* q = &m.v[(size_t)(9 * t)];
           const D3 A{q[0], q[1], q[2]}, B{q[3], q[4], q[5]}, C{q[6], q[7], q[8]};
           const D3 e1 = sub(B, A), e2 = sub(C, A);
           // glm::cross(x, y) and glm::normalize(v) = v * (1 / sqrt(dot(v, v)))
           const D3 cr{e1.y * e2.z - e2.y * e1.z, e1.z * e2.x - e2.z * e1.x,
                       e1.x * e2.y - e2.x * e1.y};
           const double inv = 1.0 / std::sqrt((cr.x * cr.x + cr.y * cr.y) + cr.z * cr.z);
-          TriDev<T> td;
           const double nrm[3] = {cr.x * inv, cr.y * inv, cr.z * inv};
+          TriGeo<T> tg;
+          TriBox<T> tb;
+          std::memset(&tb, 0, sizeof(tb));
           for (int a = 0; a < 3; ++a) {
-            td.a[a] = (T)q[a];
-            td.b[a] = (T)q[3 + a];
-            td.c[a] = (T)q[6 + a];
-            td.n[a] = (T)nrm[a];
-            // glm::min(glm::min(a, b), c) / glm::max(glm::max(a, b), c) over the T corners
-            T lo = (td.b[a] < td.a[a]) ? td.b[a] : td.a[a];
-            lo = (td.c[a] < lo) ? td.c[a] : lo;
-            T hi = (td.a[a] < td.b[a]) ? td.b[a] : td.a[a];
-            hi = (hi < td.c[a]) ? td.c[a] : hi;
-            td.lo[a] = lo;
-            td.hi[a] = hi;
+            tg.a[a] = (T)q[a];
+            tg.b[a] = (T)q[3 + a];
+            tg.c[a] = (T)q[6 + a];
+            tg.n[a] = (T)nrm[a];
+            T lo = (tg.b[a] < tg.a[a]) ? tg.b[a] : tg.a[a];
+            lo = (tg.c[a] < lo) ? tg.c[a] : lo;
+            T hi = (tg.a[a] < tg.b[a]) ? tg.b[a] : tg.a[a];
+            hi = (hi < tg.c[a]) ? tg.c[a] : hi;
+            tb.lo[a] = lo;
+            tb.hi[a] = hi;
           }
-          tris.push_back(td);
+          tb.tri = tri_base + (int32_t)t;
+          tgeo.push_back(tg);
+          boxes[(size_t)t] = tb;
         }
-        d.mesh = (int32_t)md.size();
-        md.push_back(mdv);
+        const int32_t entry_base = (int32_t)tbox.size();
+        for (size_t c2 = 0; c2 + 1 < m.off.size(); ++c2) toff.push_back(entry_base + m.off[c2]);
+        for (int32_t t : m.list) tbox.push_back(boxes[(size_t)t]);
+        toff.push_back((int32_t)tbox.size());  // end of this scene's last cell
       }
     }
-    if (!toff.empty()) toff.push_back((int32_t)ttri.size());
     ncells = cells;
     cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
     ntiles = (int32_t)(cpad / kScanTile);
@@ -436,31 +412,31 @@ struct SimImpl final : SimBase {
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1) * 4);
     tile_sums.alloc(((size_t)ntiles + 1) * 4);
+    tile_pre.alloc(((size_t)ntiles + 1) * 4);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
     scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
-    meshes_d.alloc(std::max<size_t>(md.size(), 1) * sizeof(MeshDev));
-    tris_d.alloc(std::max<size_t>(tris.size(), 1) * sizeof(TriDev<T>));
+    tgeo_d.alloc(std::max<size_t>(tgeo.size(), 1) * sizeof(TriGeo<T>));
+    tbox_d.alloc(std::max<size_t>(tbox.size(), 1) * sizeof(TriBox<T>));
     tcell_off_d.alloc(std::max<size_t>(toff.size(), 1) * 4);
-    tcell_tri_d.alloc(std::max<size_t>(ttri.size(), 1) * 4);
 
     PSX_HIP(hipMemcpy(pos[0].p, p4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
     PSX_HIP(hipMemcpy(vel[0].p, v4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
     PSX_HIP(hipMemcpy(id[0].p, ids.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
     PSX_HIP(hipMemset(count.p, 0, count.bytes));
     PSX_HIP(hipMemset(offset.p, 0, offset.bytes));
+    PSX_HIP(hipMemset(tile_sums.p, 0, tile_sums.bytes));
+    PSX_HIP(hipMemset(tile_pre.p, 0, tile_pre.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
     PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
-    if (!md.empty())
-      PSX_HIP(hipMemcpy(meshes_d.p, md.data(), md.size() * sizeof(MeshDev), hipMemcpyHostToDevice));
-    if (!tris.empty())
-      PSX_HIP(hipMemcpy(tris_d.p, tris.data(), tris.size() * sizeof(TriDev<T>), hipMemcpyHostToDevice));
+    if (!tgeo.empty())
+      PSX_HIP(hipMemcpy(tgeo_d.p, tgeo.data(), tgeo.size() * sizeof(TriGeo<T>), hipMemcpyHostToDevice));
+    if (!tbox.empty())
+      PSX_HIP(hipMemcpy(tbox_d.p, tbox.data(), tbox.size() * sizeof(TriBox<T>), hipMemcpyHostToDevice));
     if (!toff.empty())
       PSX_HIP(hipMemcpy(tcell_off_d.p, toff.data(), toff.size() * 4, hipMemcpyHostToDevice));
-    if (!ttri.empty())
-      PSX_HIP(hipMemcpy(tcell_tri_d.p, ttri.data(), ttri.size() * 4, hipMemcpyHostToDevice));
-    tri_count = (int64_t)tris.size();
-    tcell_entries = (int64_t)ttri.size();
+    tri_count = (int64_t)tgeo.size();
+    tcell_entries = (int64_t)tbox.size();
     int nb;
     table(tuning.block_integrate, &nb);
     table(tuning.block_pairs, &nb);
@@ -484,17 +460,18 @@ struct SimImpl final : SimBase {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
                                  scenes_d.as<SceneDev<T>>(), pos[a].as<Vec4<T>>(),
                                  vel[vc].as<Vec4<T>>(), cell_of.as<int32_t>(), slot.as<uint32_t>(),
-                                 count.as<uint32_t>(), meshes_d.as<MeshDev>(), tris_d.as<TriDev<T>>(),
-                                 tcell_off_d.as<int32_t>(), tcell_tri_d.as<int32_t>(),
-                                 flags.as<uint32_t>(), mode);
+                                 count.as<uint32_t>(), tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
+                                 tcell_off_d.as<int32_t>(), flags.as<uint32_t>(), mode);
     });
     launch(1, prof, [&] {
       return launch_scan_reduce(stream, count.as<uint32_t>(), tile_sums.as<uint32_t>(), ntiles);
     });
-    launch(2, prof, [&] { return launch_scan_tiles(stream, tile_sums.as<uint32_t>(), ntiles); });
+    launch(2, prof, [&] {
+      return launch_scan_tiles(stream, tile_sums.as<uint32_t>(), tile_pre.as<uint32_t>(), ntiles);
+    });
     launch(3, prof, [&] {
       return launch_scan_down(stream, count.as<uint32_t>(), offset.as<uint32_t>(),
-                              tile_sums.as<uint32_t>(), ntiles);
+                              tile_pre.as<uint32_t>(), ntiles);
     });
     launch(4, prof, [&] {
       return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
@@ -868,6 +845,7 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
     if (b < 64 || b > 1024 || (b % 64) != 0)
       return set_error(PSX_ERR_INVALID, "block sizes must be multiples of 64 in [64, 1024]");
   if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
+  if (c.pair_variant != 0) return set_error(PSX_ERR_INVALID, "pair_variant must be 0");
   PSX_TRY({
     sim->impl->retune(c);
     return PSX_OK;

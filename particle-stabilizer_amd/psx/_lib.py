@@ -44,7 +44,8 @@ class psx_scene(C.Structure):
 
 class psx_tuning(C.Structure):
     _fields_ = [("block_integrate", C.c_int32), ("block_sort", C.c_int32),
-                ("block_pairs", C.c_int32), ("use_graph", C.c_int32)]
+                ("block_pairs", C.c_int32), ("use_graph", C.c_int32),
+                ("pair_variant", C.c_int32)]
 
 
 _DP = C.POINTER(C.c_double)
@@ -76,6 +77,11 @@ _SIGNATURES = {
     "psx_debug_bucketing": (C.c_int, [C.c_void_p, _I32P, _U32P, _U32P, _I32P]),
     "psx_profile": (C.c_int, [C.c_void_p, C.c_int64, _DP, _DP, C.c_int32, _I32P]),
     "psx_kernel_name": (C.c_char_p, [C.c_int32]),
+    "psx_read_csv": (C.c_int, [C.c_char_p, C.c_double, _DP, _I64P, C.POINTER(_DP), C.POINTER(_DP),
+                               C.POINTER(_DP), C.POINTER(_DP), _I64P]),
+    "psx_write_csv": (C.c_int, [C.c_char_p, C.c_int64, _DP, _DP]),
+    "psx_read_obj": (C.c_int, [C.c_char_p, _I64P, C.POINTER(_DP)]),
+    "psx_free": (None, [C.c_void_p]),
     "psx_last_error": (C.c_char_p, []),
     "psx_abi_version": (C.c_int, []),
 }

@@ -122,31 +122,3 @@ def write_obj(path: str | os.PathLike, tris: np.ndarray, name: str = "container"
                 f.write("v %.17g %.17g %.17g\n" % (row[3 * k], row[3 * k + 1], row[3 * k + 2]))
         for i in range(t.shape[0]):
             f.write(f"f {3 * i + 1} {3 * i + 2} {3 * i + 3}\n")
-
-
-def read_obj(path: str | os.PathLike) -> np.ndarray:
-    """Reads v / f records (f tokens may be v, v/vt, v//vn, v/vt/vn; negative indices allowed).
-
-    Faces with more than three corners are split as a fan from their first corner. The result is
-    the de-indexed triangle soup in face order (PolygonContainer::readFromFile,
-    PolygonContainer.cpp:401-416).
-    """
-    verts = []
-    tris = []
-    with open(path) as f:
-        for line in f:
-            tok = line.split()
-            if not tok:
-                continue
-            if tok[0] == "v":
-                verts.append([float(tok[1]), float(tok[2]), float(tok[3])])
-            elif tok[0] == "f":
-                idx = []
-                for t in tok[1:]:
-                    k = int(t.split("/")[0])
-                    idx.append(k - 1 if k > 0 else len(verts) + k)
-                for j in range(1, len(idx) - 1):
-                    tris.append(verts[idx[0]] + verts[idx[j]] + verts[idx[j + 1]])
-    if not tris:
-        raise ValueError(f"{path}: no faces")
-    return np.asarray(tris, dtype=np.float64)

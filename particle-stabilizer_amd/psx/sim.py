@@ -280,7 +280,7 @@ def run_simulate(particles, box_min, box_max, container, gravity, walls, spring,
     mesh = None
     if container is not None and not (isinstance(container, str) and container == ""):
         if isinstance(container, (str, bytes)) or hasattr(container, "__fspath__"):
-            from .meshes import read_obj
+            from .io import read_obj
             mesh = read_obj(container)
         else:
             mesh = np.asarray(container, np.float64)

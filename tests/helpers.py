@@ -48,11 +48,11 @@ def oracle_run(oracle, sc: Scene, steps: int, dtype=None):
                       mesh=sc.mesh, dtype=dtype)
 
 
-def hip_run(sc: Scene, steps: int, graph=False):
+def hip_run(sc: Scene, steps: int, graph=False, **tuning):
     from psx import Simulation
     with Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
-        if graph:
-            s.set_tuning(use_graph=1)
+        if graph or tuning:
+            s.set_tuning(use_graph=1 if graph else 0, **tuning)
         s.step(steps)
         s.synchronize()
         return s.state()

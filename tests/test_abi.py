@@ -18,8 +18,8 @@ HEADER = REPO / "include" / "psx.h"
 
 def declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|const char\*)\s+(psx_\w+)\(", text,
-                                 flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|void|const char\*)\s+(psx_\w+)\(",
+                                 text, flags=re.M)))
 
 
 def test_library_exports_every_declared_symbol():

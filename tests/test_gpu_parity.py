@@ -53,12 +53,18 @@ def test_bucketing_edge_positions(psx_mod, oracle, dtype):
         _bucket_compare(oracle, s, pos, dtype, h_r, prm.box_min, prm.box_max)
 
 
+VARIANTS = [dict(block_pairs=256, block_integrate=256), dict(block_pairs=128, block_integrate=64),
+            dict(block_pairs=64, block_integrate=1024, block_sort=64),
+            dict(block_pairs=1024, block_integrate=128, block_sort=1024)]
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("steps", [1, 60])
-def test_box_steps_bitexact(psx_mod, oracle, dtype, steps):
+@pytest.mark.parametrize("tune", range(len(VARIANTS)))
+def test_box_steps_bitexact(psx_mod, oracle, dtype, steps, tune):
     sc = dense_box(dtype=dtype)
     ref_p, ref_v = oracle_run(oracle, sc, steps)
-    p, v = hip_run(sc, steps)
+    p, v = hip_run(sc, steps, **VARIANTS[tune])
     assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
     assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
 
@@ -92,6 +98,30 @@ def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
     p, v = hip_run(sc, steps)
     assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
     assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_mesh_fan_center_many_hits(psx_mod, oracle, dtype):
+    """Spheres resting on the centre of a 128-sliver fan floor: cell lists of ~128 triangles (the
+    wave-cooperative path) and particles touching dozens of triangles at once."""
+    from psx import meshes
+    from psx.sim import Params, Scene, mass_of
+    rng = np.random.default_rng(9)
+    mesh = meshes.open_cylinder(30.0, 30.0, 128, 2, 128)
+    n = 400
+    p = np.concatenate([rng.uniform([-2, -2, 0.1], [2, 2, 0.6], size=(n // 2, 3)),
+                        rng.uniform([-12, -12, 0.2], [12, 12, 3.0], size=(n // 2, 3))])
+    r = rng.uniform(0.3, 0.6, n)
+    v = rng.normal(0.0, 1.0, size=(n, 3))
+    v[:, 2] -= 3.0
+    prm = Params(gravity=(0, 0, -9.8), walls=(0.4,) * 6, spring=80.0, restitution=0.5, dt=1e-3,
+                 dtype=dtype)
+    sc = Scene(p, v, r, prm, mass_of(r), mesh)
+    for steps in (1, 40):
+        ref_p, ref_v = oracle_run(oracle, sc, steps)
+        hp, hv = hip_run(sc, steps)
+        assert_bitwise(as_dtype(hp, dtype), ref_p, f"pos after {steps}")
+        assert_bitwise(as_dtype(hv, dtype), ref_v, f"vel after {steps}")
 
 
 def test_state_roundtrip(psx_mod):
