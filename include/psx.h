@@ -144,6 +144,27 @@ int psx_get_grid(const psx_sim* sim, int32_t scene, double* min3, double* h, int
 int psx_debug_bucketing(psx_sim* sim, int32_t* cell, uint32_t* count, uint32_t* offset,
                         int32_t* perm);
 
+/* ---- differentiable windows ---------------------------------------------------------------- */
+/* No counterpart in the reference. Gradients of a loss on the state after a window of steps with
+ * respect to the initial state and to spring, sphere restitution and gravity_z, holding each step's
+ * contact sets (pair contacts, wall and triangle hits) fixed. */
+
+/* Replaces the coefficients of one scene (gravity, walls, spring, restitution, dt); the container
+ * bounds and dtype must be unchanged. */
+int psx_set_params(psx_sim* sim, int32_t scene, const psx_params* params);
+
+/* Runs nsteps steps from the current state, keeping a copy of the state every checkpoint_every
+ * steps in device memory for psx_window_backward. */
+int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every);
+
+/* Adjoint of the last forward window. dl_dpos/dl_dvel: dL/d(final positions, velocities), n x 3 of
+ * the handle's dtype in device memory, original order. Writes dL/d(initial positions, velocities)
+ * to dl_dpos0/dl_dvel0 (same layout) and, for each scene s, dL/d(spring, restitution, gravity_z) to
+ * dl_dparams[3s .. 3s+2] (host). Each segment between checkpoints is recomputed forward. The state
+ * afterwards is the window's final state. */
+int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, void* dl_dpos0,
+                        void* dl_dvel0, double* dl_dparams);
+
 /* ---- measurement --------------------------------------------------------------------------- */
 
 /* Runs nsteps steps with a HIP event pair around every kernel launch on the handle's stream and

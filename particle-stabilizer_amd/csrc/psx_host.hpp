@@ -94,6 +94,10 @@ struct SimBase {
   virtual void set_state_device(const void* pos, const void* vel) = 0;
   virtual void debug_bucketing(int32_t* cell, uint32_t* count, uint32_t* offset, int32_t* perm) = 0;
   virtual uint32_t read_flags() = 0;
+  virtual void set_params(int32_t scene, const psx_params& p) = 0;
+  virtual void window_forward(int64_t nsteps, int64_t every) = 0;
+  virtual void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0,
+                               double* gparams) = 0;
 };
 
 std::unique_ptr<SimBase> make_sim(int dtype);

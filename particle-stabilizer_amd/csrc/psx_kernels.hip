@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(1024)
                   const uint32_t* __restrict__ offset, const int32_t* __restrict__ tmp,
                   const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
                   const int32_t* __restrict__ id, Vec4<T>* __restrict__ pos_o,
-                  Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o) {
+                  Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
+                  int32_t* __restrict__ cell_o) {
   const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
   if (p >= n) return;
   const int32_t c = cell_of[p];
@@ -379,6 +380,7 @@ __global__ void __launch_bounds__(1024)
   pos_o[dst] = pos[p];
   vel_o[dst] = vel[p];
   id_o[dst] = me;
+  if (cell_o) cell_o[dst] = c;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -577,10 +579,10 @@ template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
                               const uint32_t* offset, const int32_t* tmp, const Vec4<T>* pos,
                               const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
-                              Vec4<T>* vel_o, int32_t* id_o) {
+                              Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o) {
   const int32_t tiles = blocks_for(n, block);
   hipLaunchKernelGGL(k_rank_gather<T>, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles,
-                     cell_of, offset, tmp, pos, vel, id, pos_o, vel_o, id_o);
+                     cell_of, offset, tmp, pos, vel, id, pos_o, vel_o, id_o, cell_o);
   return hipGetLastError();
 }
 
@@ -617,7 +619,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,          \
                                             const uint32_t*, const int32_t*, const Vec4<T>*,   \
                                             const Vec4<T>*, const int32_t*, Vec4<T>*, Vec4<T>*,\
-                                            int32_t*);                                         \
+                                            int32_t*, int32_t*);                               \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, const BlockInfo*,                 \
                                       const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
                                       const uint32_t*, Vec4<T>*);                              \

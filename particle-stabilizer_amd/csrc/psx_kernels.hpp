@@ -34,7 +34,7 @@ template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
                               const uint32_t* offset, const int32_t* tmp, const Vec4<T>* pos,
                               const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
-                              Vec4<T>* vel_o, int32_t* id_o);
+                              Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o);
 
 template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,

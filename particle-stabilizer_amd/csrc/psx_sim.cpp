@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "psx.h"
+#include "psx_adjoint.hpp"
 #include "psx_kernels.hpp"
 #include "psx_host.hpp"
 
@@ -329,25 +330,10 @@ struct SimImpl final : SimBase {
       for (int a = 0; a < 3; ++a) {
         d.grid.min[a] = sh.grid.min[a];
         d.grid.n[a] = (int32_t)sh.grid.n[a];
-        d.bmin[a] = (T)sh.prm.box_min[a];
-        d.bmax[a] = (T)sh.prm.box_max[a];
-        const T ga = (T)sh.prm.gravity[a];
-        const T dta = (T)sh.prm.dt;
-        d.gdt[a] = ga * dta;
       }
       d.grid.h = sh.grid.h;
       d.grid.cell_base = (int32_t)sh.cell_base;
-      // walls: zMin, zMax, xMin, xMax, yMin, yMax
-      d.emin[0] = (T)sh.prm.walls[2];
-      d.emax[0] = (T)sh.prm.walls[3];
-      d.emin[1] = (T)sh.prm.walls[4];
-      d.emax[1] = (T)sh.prm.walls[5];
-      d.emin[2] = (T)sh.prm.walls[0];
-      d.emax[2] = (T)sh.prm.walls[1];
-      d.dt = (T)sh.prm.dt;
-      d.spring = (T)sh.prm.spring;
-      d.es = (T)sh.prm.restitution;
-      d.ew = (T)sh.prm.walls[0];
+      fill_coefficients(d, sh.prm);
       d.rmax = (T)rmax;
       d.p_begin = (int32_t)sh.p_begin;
       d.p_end = (int32_t)(sh.p_begin + sh.n);
@@ -451,17 +437,20 @@ struct SimImpl final : SimBase {
     if (prof) PSX_HIP(hipEventRecord(prof->end(k), stream));
   }
 
-  void bucket_and_sort(int mode, Profiler* prof) {
-    const int a = cur_pos, vc = cur_vel;
+  // Bucketing of (ps, vs, is): k_integrate in `mode` (in place on ps/vs when it integrates or
+  // applies the container), scan into `off`, stable scatter, gather into (pd, vd, idd); the bucket
+  // cell of every sorted slot goes to cell_sorted when it is not NULL.
+  void bucket(Vec4<T>* ps, Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd, int32_t* idd,
+              uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof) {
     int nb_i;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
     const int32_t nn = (int32_t)n;
     launch(0, prof, [&] {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
-                                 scenes_d.as<SceneDev<T>>(), pos[a].as<Vec4<T>>(),
-                                 vel[vc].as<Vec4<T>>(), cell_of.as<int32_t>(), slot.as<uint32_t>(),
-                                 count.as<uint32_t>(), tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
-                                 tcell_off_d.as<int32_t>(), flags.as<uint32_t>(), mode);
+                                 scenes_d.as<SceneDev<T>>(), ps, vs, cell_of.as<int32_t>(),
+                                 slot.as<uint32_t>(), count.as<uint32_t>(), tgeo_d.as<TriGeo<T>>(),
+                                 tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
+                                 flags.as<uint32_t>(), mode);
     });
     launch(1, prof, [&] {
       return launch_scan_reduce(stream, count.as<uint32_t>(), tile_sums.as<uint32_t>(), ntiles);
@@ -470,26 +459,44 @@ struct SimImpl final : SimBase {
       return launch_scan_tiles(stream, tile_sums.as<uint32_t>(), tile_pre.as<uint32_t>(), ntiles);
     });
     launch(3, prof, [&] {
-      return launch_scan_down(stream, count.as<uint32_t>(), offset.as<uint32_t>(),
-                              tile_pre.as<uint32_t>(), ntiles);
+      return launch_scan_down(stream, count.as<uint32_t>(), off, tile_pre.as<uint32_t>(), ntiles);
     });
     launch(4, prof, [&] {
       return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
-                                slot.as<uint32_t>(), offset.as<uint32_t>(), id[a].as<int32_t>(),
-                                tmp.as<int32_t>());
+                                slot.as<uint32_t>(), off, is, tmp.as<int32_t>());
     });
     launch(5, prof, [&] {
-      return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
-                                   offset.as<uint32_t>(), tmp.as<int32_t>(), pos[a].as<Vec4<T>>(),
-                                   vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(),
-                                   pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
-                                   id[1 - a].as<int32_t>());
+      return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(), off,
+                                   tmp.as<int32_t>(), ps, vs, is, pd, vd, idd, cell_sorted);
     });
   }
 
-  void step_once(Profiler* prof) {
+  void bucket_and_sort(int mode, Profiler* prof, int32_t* cell_sorted = nullptr) {
     const int a = cur_pos, vc = cur_vel;
-    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof);
+    bucket(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
+           vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offset.as<uint32_t>(), cell_sorted,
+           mode, prof);
+  }
+
+  // One step; with rec >= 0 the pre-step state and the bucket-sorted post-container snapshot are
+  // copied into record slot rec for the adjoint.
+  void step_once(Profiler* prof, int64_t rec = -1) {
+    const int a = cur_pos, vc = cur_vel;
+    const size_t nn = (size_t)n;
+    if (rec >= 0) {
+      PSX_HIP(hipMemcpyAsync(rec_pre_pos.as<Vec4<T>>() + rec * nn, pos[a].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+      PSX_HIP(hipMemcpyAsync(rec_pre_vel.as<Vec4<T>>() + rec * nn, vel[vc].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+      PSX_HIP(hipMemcpyAsync(rec_pre_id.as<int32_t>() + rec * nn, id[a].p, nn * 4, hipMemcpyDeviceToDevice, stream));
+    }
+    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof,
+                    rec >= 0 ? rec_s_cell.as<int32_t>() + rec * nn : nullptr);
+    if (rec >= 0) {
+      const size_t nc = (size_t)ncells + 1;
+      PSX_HIP(hipMemcpyAsync(rec_s_pos.as<Vec4<T>>() + rec * nn, pos[1 - a].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+      PSX_HIP(hipMemcpyAsync(rec_s_vel.as<Vec4<T>>() + rec * nn, vel[1 - vc].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+      PSX_HIP(hipMemcpyAsync(rec_s_id.as<int32_t>() + rec * nn, id[1 - a].p, nn * 4, hipMemcpyDeviceToDevice, stream));
+      PSX_HIP(hipMemcpyAsync(rec_s_off.as<uint32_t>() + rec * nc, offset.p, nc * 4, hipMemcpyDeviceToDevice, stream));
+    }
     int nb_p;
     const BlockInfo* bt_p = table(tuning.block_pairs, &nb_p);
     launch(6, prof, [&] {
@@ -498,6 +505,146 @@ struct SimImpl final : SimBase {
                              offset.as<uint32_t>(), vel[vc].as<Vec4<T>>());
     });
     cur_pos = 1 - a;
+  }
+
+  // ---- differentiable windows --------------------------------------------------------------
+  int64_t win_steps = 0, win_every = 0, win_nseg = 0;
+  DevBuf ck_pos, ck_vel, ck_id;                               // (segments + 1) saved states
+  DevBuf rec_pre_pos, rec_pre_vel, rec_pre_id;                // per recorded step: pre-step state
+  DevBuf rec_s_pos, rec_s_vel, rec_s_id, rec_s_cell, rec_s_off;  // bucket-sorted snapshot
+  DevBuf posH, velH, idH, offH;                               // snapshot sorted by home cell
+  DevBuf lxA, lvA, lxB, lvB, gk, ge, gg, gout;
+
+  void save_state(int64_t slot_k) {
+    const size_t nn = (size_t)n;
+    PSX_HIP(hipMemcpyAsync(ck_pos.as<Vec4<T>>() + slot_k * nn, pos[cur_pos].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(ck_vel.as<Vec4<T>>() + slot_k * nn, vel[cur_vel].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(ck_id.as<int32_t>() + slot_k * nn, id[cur_pos].p, nn * 4, hipMemcpyDeviceToDevice, stream));
+  }
+
+  void restore_state(int64_t slot_k) {
+    const size_t nn = (size_t)n;
+    PSX_HIP(hipMemcpyAsync(pos[0].p, ck_pos.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(vel[0].p, ck_vel.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(id[0].p, ck_id.as<int32_t>() + slot_k * nn, nn * 4, hipMemcpyDeviceToDevice, stream));
+    cur_pos = 0;
+    cur_vel = 0;
+  }
+
+  void window_forward(int64_t nsteps, int64_t every) override {
+    const size_t nn = (size_t)n;
+    const int64_t nseg = (nsteps + every - 1) / every;
+    if (nseg != win_nseg || every != win_every) {
+      ck_pos.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
+      ck_vel.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
+      ck_id.alloc((size_t)(nseg + 1) * nn * 4);
+      rec_pre_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rec_pre_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rec_pre_id.alloc((size_t)every * nn * 4);
+      rec_s_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rec_s_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rec_s_id.alloc((size_t)every * nn * 4);
+      rec_s_cell.alloc((size_t)every * nn * 4);
+      rec_s_off.alloc((size_t)every * ((size_t)ncells + 1) * 4);
+      posH.alloc(nn * sizeof(Vec4<T>));
+      velH.alloc(nn * sizeof(Vec4<T>));
+      idH.alloc(nn * 4);
+      offH.alloc(((size_t)cpad + 1) * 4);
+      for (DevBuf* b : {&lxA, &lvA, &lxB, &lvB}) b->alloc(3 * nn * sizeof(T));
+      for (DevBuf* b : {&gk, &ge, &gg}) b->alloc(nn * sizeof(T));
+      gout.alloc(3 * sizeof(double) * scenes.size());
+    }
+    win_steps = nsteps;
+    win_every = every;
+    win_nseg = nseg;
+    for (int64_t sgm = 0; sgm < nseg; ++sgm) {
+      save_state(sgm);
+      step(std::min(every, nsteps - sgm * every));
+    }
+    save_state(nseg);
+  }
+
+  void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0, double* gparams) override {
+    if (win_nseg <= 0) throw Error(PSX_ERR_STATE, "psx_window_backward needs a psx_window_forward first");
+    const size_t nn = (size_t)n;
+    const size_t nc = (size_t)ncells + 1;
+    PSX_HIP(hipMemcpyAsync(lxA.p, dlx, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(lvA.p, dlv, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    for (DevBuf* b : {&gk, &ge, &gg}) PSX_HIP(launch_fill<T>(stream, (int64_t)nn, b->as<T>(), T(0)));
+    int nb;
+    const BlockInfo* bt = table(256, &nb);
+    for (int64_t sgm = win_nseg - 1; sgm >= 0; --sgm) {
+      restore_state(sgm);
+      const int64_t ks = std::min(win_every, win_steps - sgm * win_every);
+      for (int64_t k = 0; k < ks; ++k) step_once(nullptr, k);
+      for (int64_t k = ks - 1; k >= 0; --k) {
+        Vec4<T>* sp = rec_s_pos.as<Vec4<T>>() + k * nn;
+        Vec4<T>* sv = rec_s_vel.as<Vec4<T>>() + k * nn;
+        const int32_t* si = rec_s_id.as<int32_t>() + k * nn;
+        // home-cell bucketing of the snapshot (hash only: nothing is written back to sp/sv)
+        bucket(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+               offH.as<uint32_t>(), nullptr, 0, nullptr);
+        PSX_HIP(launch_adj_pairs<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
+                                    rec_s_off.as<uint32_t>() + k * nc, rec_s_cell.as<int32_t>() + k * nn,
+                                    posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+                                    offH.as<uint32_t>(), lxA.as<T>(), lvA.as<T>(), lxB.as<T>(),
+                                    lvB.as<T>(), gk.as<T>(), ge.as<T>()));
+        PSX_HIP(launch_adj_container<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(),
+                                        rec_pre_pos.as<Vec4<T>>() + k * nn,
+                                        rec_pre_vel.as<Vec4<T>>() + k * nn,
+                                        rec_pre_id.as<int32_t>() + k * nn, tgeo_d.as<TriGeo<T>>(),
+                                        tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
+                                        lxB.as<T>(), lvB.as<T>(), lxA.as<T>(), lvA.as<T>(), gg.as<T>()));
+      }
+    }
+    PSX_HIP(hipMemcpyAsync(dlx0, lxA.p, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(dlv0, lvA.p, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    for (size_t sc = 0; sc < scenes.size(); ++sc) {
+      const int64_t b0 = scenes[sc].p_begin;
+      PSX_HIP(launch_reduce3<T>(stream, (int32_t)scenes[sc].n, gk.as<T>() + b0, ge.as<T>() + b0,
+                                gg.as<T>() + b0, gout.as<double>() + 3 * sc));
+    }
+    std::vector<double> h(3 * scenes.size());
+    PSX_HIP(hipMemcpyAsync(h.data(), gout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+    if (gparams) std::memcpy(gparams, h.data(), h.size() * sizeof(double));
+    restore_state(win_nseg);  // back to the window's final state
+    (void)nc;
+  }
+
+  void set_params(int32_t scene, const psx_params& p) override {
+    SceneHost& sh = scenes[(size_t)scene];
+    for (int a = 0; a < 3; ++a)
+      if (p.box_min[a] != sh.prm.box_min[a] || p.box_max[a] != sh.prm.box_max[a])
+        throw Error(PSX_ERR_INVALID, "psx_set_params cannot change the container bounds");
+    if (p.dtype != sh.prm.dtype) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change dtype");
+    sh.prm = p;
+    SceneDev<T> d;
+    PSX_HIP(hipStreamSynchronize(stream));
+    PSX_HIP(hipMemcpy(&d, scenes_d.as<SceneDev<T>>() + scene, sizeof(d), hipMemcpyDeviceToHost));
+    fill_coefficients(d, sh.prm);
+    PSX_HIP(hipMemcpy(scenes_d.as<SceneDev<T>>() + scene, &d, sizeof(d), hipMemcpyHostToDevice));
+  }
+
+  static void fill_coefficients(SceneDev<T>& d, const psx_params& prm) {
+    for (int a = 0; a < 3; ++a) {
+      d.bmin[a] = (T)prm.box_min[a];
+      d.bmax[a] = (T)prm.box_max[a];
+      const T ga = (T)prm.gravity[a];
+      const T dta = (T)prm.dt;
+      d.gdt[a] = ga * dta;
+    }
+    // walls: zMin, zMax, xMin, xMax, yMin, yMax
+    d.emin[0] = (T)prm.walls[2];
+    d.emax[0] = (T)prm.walls[3];
+    d.emin[1] = (T)prm.walls[4];
+    d.emax[1] = (T)prm.walls[5];
+    d.emin[2] = (T)prm.walls[0];
+    d.emax[2] = (T)prm.walls[1];
+    d.dt = (T)prm.dt;
+    d.spring = (T)prm.spring;
+    d.es = (T)prm.restitution;
+    d.ew = (T)prm.walls[0];
   }
 
   void capture(int a, int vc) {
@@ -915,6 +1062,36 @@ int psx_debug_bucketing(psx_sim* sim, int32_t* cell, uint32_t* count, uint32_t* 
   if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
   PSX_TRY({
     sim->impl->debug_bucketing(cell, count, offset, perm);
+    return PSX_OK;
+  })
+}
+
+int psx_set_params(psx_sim* sim, int32_t scene, const psx_params* params) {
+  if (!sim || !params) return set_error(PSX_ERR_INVALID, "NULL argument");
+  if (scene < 0 || scene >= (int32_t)sim->impl->scenes.size()) return set_error(PSX_ERR_INVALID, "scene out of range");
+  int rc = check_params(params);
+  if (rc) return rc;
+  PSX_TRY({
+    sim->impl->set_params(scene, *params);
+    return PSX_OK;
+  })
+}
+
+int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  if (nsteps <= 0 || checkpoint_every <= 0) return set_error(PSX_ERR_INVALID, "nsteps and checkpoint_every must be positive");
+  PSX_TRY({
+    sim->impl->window_forward(nsteps, std::min(nsteps, checkpoint_every));
+    return PSX_OK;
+  })
+}
+
+int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, void* dl_dpos0,
+                        void* dl_dvel0, double* dl_dparams) {
+  if (!sim || !dl_dpos || !dl_dvel || !dl_dpos0 || !dl_dvel0)
+    return set_error(PSX_ERR_INVALID, "NULL argument");
+  PSX_TRY({
+    sim->impl->window_backward(dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams);
     return PSX_OK;
   })
 }

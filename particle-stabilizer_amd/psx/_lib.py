@@ -77,6 +77,10 @@ _SIGNATURES = {
     "psx_debug_bucketing": (C.c_int, [C.c_void_p, _I32P, _U32P, _U32P, _I32P]),
     "psx_profile": (C.c_int, [C.c_void_p, C.c_int64, _DP, _DP, C.c_int32, _I32P]),
     "psx_kernel_name": (C.c_char_p, [C.c_int32]),
+    "psx_set_params": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(psx_params)]),
+    "psx_window_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "psx_window_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      _DP]),
     "psx_read_csv": (C.c_int, [C.c_char_p, C.c_double, _DP, _I64P, C.POINTER(_DP), C.POINTER(_DP),
                                C.POINTER(_DP), C.POINTER(_DP), _I64P]),
     "psx_write_csv": (C.c_int, [C.c_char_p, C.c_int64, _DP, _DP]),

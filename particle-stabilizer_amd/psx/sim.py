@@ -206,6 +206,26 @@ class Simulation:
             perm.ctypes.data_as(C.POINTER(C.c_int32))))
         return {"cell": cell, "count": count, "offset": offset, "perm": perm}
 
+    def set_params(self, params: "Params", scene: int = 0) -> None:
+        """Replaces a scene's coefficients (gravity, walls, spring, restitution, dt)."""
+        cp = params.to_c()
+        check(self._lib.psx_set_params(self._h, scene, C.byref(cp)))
+        if scene == 0:
+            self.params = params
+
+    def window_forward(self, nsteps: int, checkpoint_every: int = 100) -> None:
+        check(self._lib.psx_window_forward(self._h, int(nsteps), int(checkpoint_every)))
+
+    def window_backward(self, dl_dpos_ptr: int, dl_dvel_ptr: int, out_pos_ptr: int,
+                        out_vel_ptr: int) -> np.ndarray:
+        """Device pointers (n x 3, the handle's dtype); returns dL/d(spring, restitution, g_z)
+        per scene as an (n_scenes, 3) array."""
+        g = np.zeros(3 * self.num_scenes)
+        check(self._lib.psx_window_backward(self._h, C.c_void_p(dl_dpos_ptr),
+                                            C.c_void_p(dl_dvel_ptr), C.c_void_p(out_pos_ptr),
+                                            C.c_void_p(out_vel_ptr), _dptr(g)))
+        return g.reshape(-1, 3)
+
     def profile(self, nsteps: int) -> dict:
         """Mean device time (ms) per launch of each kernel over nsteps steps (HIP events)."""
         ms = np.zeros(32)
