@@ -55,6 +55,134 @@ def cpu_baseline(sc, steps: int) -> dict:
                       f"mesh loop), {dt:.2f} s"}
 
 
+def _timed(sim, steps: int) -> float:
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_hopper(steps: int = 100, warmup: int = 10) -> dict:
+    """The 4,194,304-sphere hopper settle (8,192-triangle mesh), float32."""
+    import psx
+    from psx import scenes
+    sc = scenes.hopper_settle()
+    n = sc.radius.shape[0]
+    sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
+    sim.set_tuning(use_graph=1)
+    sim.step(warmup)
+    p0, v0 = sim.state()
+    dt = _timed(sim, steps)
+    sim.set_state(p0, v0)
+    prof = sim.profile(min(steps, 50))
+    algo = algorithmic_bytes(n, sim.num_cells, 4)
+    kern = {k: {"ms": round(d["ms"], 4),
+                "GBps": round(algo[k] / (d["ms"] * 1e-3) / 1e9, 1) if algo.get(k) else None}
+            for k, d in prof.items()}
+    sim.close()
+    pm = prof["k_pairs"]["ms"]
+    return {"workload": f"hopper_settle: {n:,} spheres, 8,192-triangle mesh, f32",
+            "value": round(n * steps / dt, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(1e3 * dt / steps, 4), "steps": steps, "kernels": kern,
+            "k_pairs_frac_hbm": round(algo["k_pairs"] / (pm * 1e-3) / HBM_PEAK, 4)}
+
+
+def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
+    """256 independent box scenes of uneven size: one launch per kernel per step (hipGraph)
+    against the same kernels launched once per scene, on one stream and on one stream per scene."""
+    import psx
+    from psx import scenes
+    scs = scenes.batched_boxes()
+    total = sum(sc.radius.shape[0] for sc in scs)
+    b = psx.Batch(scs)
+    b.set_tuning(use_graph=1)
+    b.step(warmup)
+    tb = _timed(b, steps)
+    cells = b.num_cells
+    b.close()
+    sims = [psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) for sc in scs]
+    import ctypes
+    for s in sims:
+        s.step(warmup)
+    for s in sims:
+        s.synchronize()
+    # (a) one stream: each scene's launches in turn, as a loop over scenes would issue them
+    st = ctypes.c_void_p()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipStreamCreate(ctypes.byref(st))
+    for s in sims:
+        s.set_stream(st.value)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sims:
+            s.step(1)
+    for s in sims:
+        s.synchronize()
+    t_one = time.perf_counter() - t0
+    # (b) one stream per scene (the scenes' launches may overlap on the GPU)
+    for s in sims:
+        s.set_stream(None)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sims:
+            s.step(1)
+    for s in sims:
+        s.synchronize()
+    t_many = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    hip.hipStreamDestroy(st)
+    rate = total * steps / tb
+    return {"workload": f"batched_boxes: 256 scenes, {total:,} spheres, {cells:,} cells, f32",
+            "value": round(rate, 1), "unit": "particle-steps/s", "steps": steps,
+            "ms_per_step": round(1e3 * tb / steps, 4),
+            "per_scene_one_stream": round(total * steps / t_one, 1),
+            "per_scene_stream_each": round(total * steps / t_many, 1),
+            "speedup_vs_one_stream": round(t_one / tb, 2),
+            "speedup_vs_stream_each": round(t_many / tb, 2)}
+
+
+def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
+    """16,384 spheres in an 80-triangle box mesh, float32: forward window with checkpoints every
+    100 steps against the backward (segment recompute + adjoint)."""
+    import torch
+
+    import psx
+    from psx import scenes
+    sc = scenes.calibration_scene()
+    n = sc.radius.shape[0]
+    sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
+    sim.set_tuning(use_graph=1)
+    dev = torch.device("cuda")
+    gx = torch.randn((n, 3), device=dev, dtype=torch.float32)
+    gv = torch.randn((n, 3), device=dev, dtype=torch.float32)
+    ox = torch.empty_like(gx)
+    ov = torch.empty_like(gv)
+    torch.cuda.synchronize()
+    sim.window_forward(min(nsteps, 200), every)  # warm-up (allocations, graph capture)
+    sim.window_backward(gx.data_ptr(), gv.data_ptr(), ox.data_ptr(), ov.data_ptr())
+    sim.set_state(sc.pos, sc.vel)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.window_forward(nsteps, every)
+    sim.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = sim.window_backward(gx.data_ptr(), gv.data_ptr(), ox.data_ptr(), ov.data_ptr())
+    sim.synchronize()
+    tbk = time.perf_counter() - t0
+    st = sim.window_stats()
+    sim.close()
+    return {"workload": f"calibration_scene: {n:,} spheres, 80-triangle box mesh, f32, "
+                        f"{nsteps}-step window, checkpoint every {every}",
+            "forward_ms": round(1e3 * tf, 2), "backward_ms": round(1e3 * tbk, 2),
+            "ratio": round(tbk / tf, 3), "recompute_ms": round(st["replay_ms"], 2),
+            "adjoint_ms_per_step": round(st["adjoint_ms"] / nsteps, 5),
+            "forward_ms_per_step": round(1e3 * tf / nsteps, 5),
+            "grad_finite": bool(np.isfinite(g).all())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +195,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the hopper, batched and adjoint measurements")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,6 +259,16 @@ def main() -> int:
         "roofline": roofline,
         "kernels": kernels,
     }
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("hopper_settle", bench_hopper), ("batched", bench_batched),
+                         ("adjoint", bench_adjoint)):
+            try:
+                extras[name] = fn()
+            except Exception as e:  # report, keep the headline line
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"bench.py: {name} done", file=sys.stderr, flush=True)
+        line["extras"] = extras
     if not args.no_cpu_baseline:
         cpu_steps = args.cpu_steps or {"cylinder_settle": 20, "hopper_settle": 3,
                                        "box_settle": 400}[args.workload]

@@ -165,6 +165,9 @@ int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every);
 int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, void* dl_dpos0,
                         void* dl_dvel0, double* dl_dparams);
 
+/* Device time of the last psx_window_backward spent recomputing segments and in the adjoint steps. */
+int psx_window_stats(const psx_sim* sim, double* replay_ms, double* adjoint_ms);
+
 /* ---- measurement --------------------------------------------------------------------------- */
 
 /* Runs nsteps steps with a HIP event pair around every kernel launch on the handle's stream and

@@ -573,10 +573,21 @@ struct SimImpl final : SimBase {
     for (DevBuf* b : {&gk, &ge, &gg}) PSX_HIP(launch_fill<T>(stream, (int64_t)nn, b->as<T>(), T(0)));
     int nb;
     const BlockInfo* bt = table(256, &nb);
+    std::vector<hipEvent_t> ev((size_t)(3 * win_nseg), nullptr);
+    for (auto& e : ev) PSX_HIP(hipEventCreate(&e));
+    struct EvGuard {
+      std::vector<hipEvent_t>& v;
+      ~EvGuard() {
+        for (auto e : v)
+          if (e) (void)hipEventDestroy(e);
+      }
+    } guard{ev};
     for (int64_t sgm = win_nseg - 1; sgm >= 0; --sgm) {
       restore_state(sgm);
       const int64_t ks = std::min(win_every, win_steps - sgm * win_every);
+      PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm)], stream));
       for (int64_t k = 0; k < ks; ++k) step_once(nullptr, k);
+      PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm + 1)], stream));
       for (int64_t k = ks - 1; k >= 0; --k) {
         Vec4<T>* sp = rec_s_pos.as<Vec4<T>>() + k * nn;
         Vec4<T>* sv = rec_s_vel.as<Vec4<T>>() + k * nn;
@@ -596,6 +607,7 @@ struct SimImpl final : SimBase {
                                         tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
                                         lxB.as<T>(), lvB.as<T>(), lxA.as<T>(), lvA.as<T>(), gg.as<T>()));
       }
+      PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm + 2)], stream));
     }
     PSX_HIP(hipMemcpyAsync(dlx0, lxA.p, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
     PSX_HIP(hipMemcpyAsync(dlv0, lvA.p, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
@@ -608,9 +620,22 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpyAsync(h.data(), gout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     PSX_HIP(hipStreamSynchronize(stream));
     if (gparams) std::memcpy(gparams, h.data(), h.size() * sizeof(double));
+    win_replay_ms = win_adjoint_ms = 0.0;
+    for (int64_t sgm = 0; sgm < win_nseg; ++sgm) {
+      float a = 0.f, b = 0.f;
+      PSX_HIP(hipEventElapsedTime(&a, ev[(size_t)(3 * sgm)], ev[(size_t)(3 * sgm + 1)]));
+      PSX_HIP(hipEventElapsedTime(&b, ev[(size_t)(3 * sgm + 1)], ev[(size_t)(3 * sgm + 2)]));
+      win_replay_ms += a;
+      win_adjoint_ms += b;
+    }
     restore_state(win_nseg);  // back to the window's final state
-    (void)nc;
   }
+
+  void window_stats(double* replay_ms, double* adjoint_ms) override {
+    if (replay_ms) *replay_ms = win_replay_ms;
+    if (adjoint_ms) *adjoint_ms = win_adjoint_ms;
+  }
+  double win_replay_ms = 0.0, win_adjoint_ms = 0.0;
 
   void set_params(int32_t scene, const psx_params& p) override {
     SceneHost& sh = scenes[(size_t)scene];
@@ -1094,6 +1119,12 @@ int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, 
     sim->impl->window_backward(dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams);
     return PSX_OK;
   })
+}
+
+int psx_window_stats(const psx_sim* sim, double* replay_ms, double* adjoint_ms) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  sim->impl->window_stats(replay_ms, adjoint_ms);
+  return PSX_OK;
 }
 
 int psx_profile(psx_sim* sim, int64_t nsteps, double* kernel_ms, double* launches_per_step,

@@ -79,6 +79,7 @@ _SIGNATURES = {
     "psx_kernel_name": (C.c_char_p, [C.c_int32]),
     "psx_set_params": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(psx_params)]),
     "psx_window_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "psx_window_stats": (C.c_int, [C.c_void_p, _DP, _DP]),
     "psx_window_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       _DP]),
     "psx_read_csv": (C.c_int, [C.c_char_p, C.c_double, _DP, _I64P, C.POINTER(_DP), C.POINTER(_DP),

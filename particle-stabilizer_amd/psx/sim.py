@@ -226,6 +226,12 @@ class Simulation:
                                             C.c_void_p(out_vel_ptr), _dptr(g)))
         return g.reshape(-1, 3)
 
+    def window_stats(self) -> dict:
+        """Device time (ms) of the last backward: segment recompute and adjoint steps."""
+        a, b = C.c_double(), C.c_double()
+        check(self._lib.psx_window_stats(self._h, C.byref(a), C.byref(b)))
+        return {"replay_ms": a.value, "adjoint_ms": b.value}
+
     def profile(self, nsteps: int) -> dict:
         """Mean device time (ms) per launch of each kernel over nsteps steps (HIP events)."""
         ms = np.zeros(32)
