@@ -199,8 +199,8 @@ __device__ inline void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool
 template <class T>
 __global__ void __launch_bounds__(1024)
     k_integrate(int32_t ntab, const BlockInfo* __restrict__ btab,
-                const SceneDev<T>* __restrict__ scenes, Vec4<T>* __restrict__ pos,
-                Vec4<T>* __restrict__ vel, int32_t* __restrict__ cell_of,
+                const SceneDev<T>* __restrict__ scenes, const Vec4<T>* pos, const Vec4<T>* vel,
+                Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* __restrict__ cell_of,
                 uint32_t* __restrict__ slot, uint32_t* __restrict__ count,
                 const TriGeo<T>* __restrict__ tgeo,
                 const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
@@ -246,8 +246,8 @@ __global__ void __launch_bounds__(1024)
       if (!ok) atomicOr(flags, kFlagNonFinite);
     }
     if (mode & (kModeIntegrate | kModeContainer)) {
-      pos[p] = x;
-      vel[p] = v;
+      pos_o[p] = x;
+      vel_o[p] = v;
     }
     cell_of[p] = c;
     slot[p] = atomicAdd(count + c, 1u);
@@ -538,12 +538,13 @@ static inline int32_t blocks_for(int64_t n, int b) { return (int32_t)((n + b - 1
 
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, Vec4<T>* pos, Vec4<T>* vel,
-                            int32_t* cell_of, uint32_t* slot, uint32_t* count,
-                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
+                            const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
+                            Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of, uint32_t* slot,
+                            uint32_t* count, const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode) {
   hipLaunchKernelGGL(k_integrate<T>, dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
-                     pos, vel, cell_of, slot, count, tgeo, tbox, tcell_off, flags, mode);
+                     pos, vel, pos_o, vel_o, cell_of, slot, count, tgeo, tbox, tcell_off, flags,
+                     mode);
   return hipGetLastError();
 }
 
@@ -613,7 +614,8 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
 
 #define PSX_INSTANTIATE(T)                                                                      \
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
-                                          const SceneDev<T>*, Vec4<T>*, Vec4<T>*, int32_t*,    \
+                                          const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,  \
+                                          Vec4<T>*, Vec4<T>*, int32_t*,                        \
                                           uint32_t*, uint32_t*, const TriGeo<T>*,              \
                                           const TriBox<T>*, const int32_t*, uint32_t*, int);   \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,          \

@@ -14,8 +14,9 @@ constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, Vec4<T>* pos, Vec4<T>* vel,
-                            int32_t* cell_of, uint32_t* slot, uint32_t* count,
+                            const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
+                            Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of, uint32_t* slot,
+                            uint32_t* count,
                             const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode);
 

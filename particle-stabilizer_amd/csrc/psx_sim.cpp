@@ -253,6 +253,7 @@ struct SimImpl final : SimBase {
   }
 
   void destroy_graphs() {
+    destroy_window_graphs();
     for (auto& row : graphs)
       for (auto& g : row)
         if (g) {
@@ -437,17 +438,20 @@ struct SimImpl final : SimBase {
     if (prof) PSX_HIP(hipEventRecord(prof->end(k), stream));
   }
 
-  // Bucketing of (ps, vs, is): k_integrate in `mode` (in place on ps/vs when it integrates or
-  // applies the container), scan into `off`, stable scatter, gather into (pd, vd, idd); the bucket
-  // cell of every sorted slot goes to cell_sorted when it is not NULL.
-  void bucket(Vec4<T>* ps, Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd, int32_t* idd,
-              uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof) {
+  // Bucketing: k_integrate in `mode` reads (ps, vs) and, when it integrates or applies the
+  // container, writes the result to (po, vo) (which may alias ps, vs); scan into `off`; stable
+  // scatter of the ids `is`; gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
+  // cell_sorted when it is not NULL.
+  void bucket(const Vec4<T>* ps, const Vec4<T>* vs, Vec4<T>* po, Vec4<T>* vo, const int32_t* is,
+              Vec4<T>* pd, Vec4<T>* vd, int32_t* idd, uint32_t* off, int32_t* cell_sorted,
+              int mode, Profiler* prof) {
     int nb_i;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
     const int32_t nn = (int32_t)n;
+    const bool writes = (mode & (kModeIntegrate | kModeContainer)) != 0;
     launch(0, prof, [&] {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
-                                 scenes_d.as<SceneDev<T>>(), ps, vs, cell_of.as<int32_t>(),
+                                 scenes_d.as<SceneDev<T>>(), ps, vs, po, vo, cell_of.as<int32_t>(),
                                  slot.as<uint32_t>(), count.as<uint32_t>(), tgeo_d.as<TriGeo<T>>(),
                                  tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
                                  flags.as<uint32_t>(), mode);
@@ -467,53 +471,58 @@ struct SimImpl final : SimBase {
     });
     launch(5, prof, [&] {
       return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(), off,
-                                   tmp.as<int32_t>(), ps, vs, is, pd, vd, idd, cell_sorted);
+                                   tmp.as<int32_t>(), writes ? po : ps, writes ? vo : vs, is, pd,
+                                   vd, idd, cell_sorted);
     });
   }
 
-  void bucket_and_sort(int mode, Profiler* prof, int32_t* cell_sorted = nullptr) {
+  void bucket_and_sort(int mode, Profiler* prof) {
     const int a = cur_pos, vc = cur_vel;
-    bucket(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
-           vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offset.as<uint32_t>(), cell_sorted,
-           mode, prof);
+    Vec4<T>* p = pos[a].as<Vec4<T>>();
+    Vec4<T>* v = vel[vc].as<Vec4<T>>();
+    bucket(p, v, p, v, id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
+           id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr, mode, prof);
   }
 
-  // One step; with rec >= 0 the pre-step state and the bucket-sorted post-container snapshot are
-  // copied into record slot rec for the adjoint.
-  void step_once(Profiler* prof, int64_t rec = -1) {
-    const int a = cur_pos, vc = cur_vel;
-    const size_t nn = (size_t)n;
-    if (rec >= 0) {
-      PSX_HIP(hipMemcpyAsync(rec_pre_pos.as<Vec4<T>>() + rec * nn, pos[a].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-      PSX_HIP(hipMemcpyAsync(rec_pre_vel.as<Vec4<T>>() + rec * nn, vel[vc].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-      PSX_HIP(hipMemcpyAsync(rec_pre_id.as<int32_t>() + rec * nn, id[a].p, nn * 4, hipMemcpyDeviceToDevice, stream));
-    }
-    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof,
-                    rec >= 0 ? rec_s_cell.as<int32_t>() + rec * nn : nullptr);
-    if (rec >= 0) {
-      const size_t nc = (size_t)ncells + 1;
-      PSX_HIP(hipMemcpyAsync(rec_s_pos.as<Vec4<T>>() + rec * nn, pos[1 - a].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-      PSX_HIP(hipMemcpyAsync(rec_s_vel.as<Vec4<T>>() + rec * nn, vel[1 - vc].p, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-      PSX_HIP(hipMemcpyAsync(rec_s_id.as<int32_t>() + rec * nn, id[1 - a].p, nn * 4, hipMemcpyDeviceToDevice, stream));
-      PSX_HIP(hipMemcpyAsync(rec_s_off.as<uint32_t>() + rec * nc, offset.p, nc * 4, hipMemcpyDeviceToDevice, stream));
-    }
+  void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
+             Profiler* prof) {
     int nb_p;
     const BlockInfo* bt_p = table(tuning.block_pairs, &nb_p);
     launch(6, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, bt_p, scenes_d.as<SceneDev<T>>(),
-                             pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
-                             offset.as<uint32_t>(), vel[vc].as<Vec4<T>>());
+                             ps, vs, off, vout);
     });
+  }
+
+  void step_once(Profiler* prof) {
+    const int a = cur_pos, vc = cur_vel;
+    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof);
+    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offset.as<uint32_t>(),
+          vel[vc].as<Vec4<T>>(), prof);
     cur_pos = 1 - a;
   }
 
   // ---- differentiable windows --------------------------------------------------------------
+  // Forward: the state is saved every win_every steps (ck_*, slot nseg = final state). Backward,
+  // per segment: the segment's start state is copied to seg_*, the segment is recomputed with every
+  // step recorded (its bucket-sorted post-container snapshot rs_*, offsets rs_off, bucket cells
+  // rs_cell, and the new velocities rs_vnew), then the adjoint sweeps the steps in reverse. Step k's
+  // pre-step state is seg_* for k = 0 and (rs_pos, rs_vnew, rs_id)[k-1] otherwise, so the
+  // recompute copies nothing. Both halves are replayed from hipGraphs captured per segment length.
   int64_t win_steps = 0, win_every = 0, win_nseg = 0;
-  DevBuf ck_pos, ck_vel, ck_id;                               // (segments + 1) saved states
-  DevBuf rec_pre_pos, rec_pre_vel, rec_pre_id;                // per recorded step: pre-step state
-  DevBuf rec_s_pos, rec_s_vel, rec_s_id, rec_s_cell, rec_s_off;  // bucket-sorted snapshot
-  DevBuf posH, velH, idH, offH;                               // snapshot sorted by home cell
+  DevBuf ck_pos, ck_vel, ck_id, seg_pos, seg_vel, seg_id, tmp_pos, tmp_vel;
+  DevBuf rs_pos, rs_vel, rs_id, rs_cell, rs_off, rs_vnew;
+  DevBuf posH, velH, idH, offH;
   DevBuf lxA, lvA, lxB, lvB, gk, ge, gg, gout;
+  std::map<int64_t, hipGraphExec_t> g_replay, g_adjoint;
+  double win_replay_ms = 0.0, win_adjoint_ms = 0.0;
+
+  void destroy_window_graphs() {
+    for (auto* m : {&g_replay, &g_adjoint}) {
+      for (auto& kv : *m) (void)hipGraphExecDestroy(kv.second);
+      m->clear();
+    }
+  }
 
   void save_state(int64_t slot_k) {
     const size_t nn = (size_t)n;
@@ -522,34 +531,32 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpyAsync(ck_id.as<int32_t>() + slot_k * nn, id[cur_pos].p, nn * 4, hipMemcpyDeviceToDevice, stream));
   }
 
-  void restore_state(int64_t slot_k) {
+  void load_state(int64_t slot_k, void* dp, void* dv, void* di) {
     const size_t nn = (size_t)n;
-    PSX_HIP(hipMemcpyAsync(pos[0].p, ck_pos.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-    PSX_HIP(hipMemcpyAsync(vel[0].p, ck_vel.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
-    PSX_HIP(hipMemcpyAsync(id[0].p, ck_id.as<int32_t>() + slot_k * nn, nn * 4, hipMemcpyDeviceToDevice, stream));
-    cur_pos = 0;
-    cur_vel = 0;
+    PSX_HIP(hipMemcpyAsync(dp, ck_pos.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(dv, ck_vel.as<Vec4<T>>() + slot_k * nn, nn * sizeof(Vec4<T>), hipMemcpyDeviceToDevice, stream));
+    PSX_HIP(hipMemcpyAsync(di, ck_id.as<int32_t>() + slot_k * nn, nn * 4, hipMemcpyDeviceToDevice, stream));
   }
 
   void window_forward(int64_t nsteps, int64_t every) override {
     const size_t nn = (size_t)n;
     const int64_t nseg = (nsteps + every - 1) / every;
     if (nseg != win_nseg || every != win_every) {
+      destroy_window_graphs();
       ck_pos.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
       ck_vel.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
       ck_id.alloc((size_t)(nseg + 1) * nn * 4);
-      rec_pre_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
-      rec_pre_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
-      rec_pre_id.alloc((size_t)every * nn * 4);
-      rec_s_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
-      rec_s_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
-      rec_s_id.alloc((size_t)every * nn * 4);
-      rec_s_cell.alloc((size_t)every * nn * 4);
-      rec_s_off.alloc((size_t)every * ((size_t)ncells + 1) * 4);
-      posH.alloc(nn * sizeof(Vec4<T>));
-      velH.alloc(nn * sizeof(Vec4<T>));
+      for (DevBuf* b : {&seg_pos, &seg_vel, &tmp_pos, &tmp_vel, &posH, &velH})
+        b->alloc(nn * sizeof(Vec4<T>));
+      seg_id.alloc(nn * 4);
       idH.alloc(nn * 4);
       offH.alloc(((size_t)cpad + 1) * 4);
+      rs_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_vnew.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_id.alloc((size_t)every * nn * 4);
+      rs_cell.alloc((size_t)every * nn * 4);
+      rs_off.alloc((size_t)every * ((size_t)cpad + 1) * 4);  // k_scan_down writes cpad + 1
       for (DevBuf* b : {&lxA, &lvA, &lxB, &lvB}) b->alloc(3 * nn * sizeof(T));
       for (DevBuf* b : {&gk, &ge, &gg}) b->alloc(nn * sizeof(T));
       gout.alloc(3 * sizeof(double) * scenes.size());
@@ -564,15 +571,91 @@ struct SimImpl final : SimBase {
     save_state(nseg);
   }
 
+  // pre-step state of recorded step k
+  const Vec4<T>* pre_pos(int64_t k) {
+    return k == 0 ? seg_pos.as<Vec4<T>>() : rs_pos.as<Vec4<T>>() + (k - 1) * n;
+  }
+  const Vec4<T>* pre_vel(int64_t k) {
+    return k == 0 ? seg_vel.as<Vec4<T>>() : rs_vnew.as<Vec4<T>>() + (k - 1) * n;
+  }
+  const int32_t* pre_id(int64_t k) {
+    return k == 0 ? seg_id.as<int32_t>() : rs_id.as<int32_t>() + (k - 1) * n;
+  }
+
+  void replay_segment(int64_t ks) {
+    const size_t nn = (size_t)n, nc = (size_t)cpad + 1;  // offset record stride
+    for (int64_t k = 0; k < ks; ++k) {
+      Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
+      Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
+      uint32_t* so = rs_off.as<uint32_t>() + k * nc;
+      bucket(pre_pos(k), pre_vel(k), tmp_pos.as<Vec4<T>>(), tmp_vel.as<Vec4<T>>(), pre_id(k), sp, sv,
+             rs_id.as<int32_t>() + k * nn, so, rs_cell.as<int32_t>() + k * nn,
+             kModeIntegrate | kModeContainer | kModeCheck, nullptr);
+      pairs(sp, sv, so, rs_vnew.as<Vec4<T>>() + k * nn, nullptr);
+    }
+  }
+
+  void adjoint_segment(int64_t ks) {
+    const size_t nn = (size_t)n, nc = (size_t)cpad + 1;  // offset record stride
+    int nb;
+    const BlockInfo* bt = table(256, &nb);
+    for (int64_t k = ks - 1; k >= 0; --k) {
+      Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
+      Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
+      const int32_t* si = rs_id.as<int32_t>() + k * nn;
+      // home-cell bucketing of the snapshot (hash only: nothing is written to sp/sv)
+      bucket(sp, sv, sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+             offH.as<uint32_t>(), nullptr, 0, nullptr);
+      PSX_HIP(launch_adj_pairs<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
+                                  rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,
+                                  posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+                                  offH.as<uint32_t>(), lxA.as<T>(), lvA.as<T>(), lxB.as<T>(),
+                                  lvB.as<T>(), gk.as<T>(), ge.as<T>()));
+      PSX_HIP(launch_adj_container<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), pre_pos(k),
+                                      pre_vel(k), pre_id(k), tgeo_d.as<TriGeo<T>>(),
+                                      tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
+                                      lxB.as<T>(), lvB.as<T>(), lxA.as<T>(), lvA.as<T>(), gg.as<T>()));
+    }
+  }
+
+  template <class F>
+  hipGraphExec_t capture_graph(F&& f) {
+    hipGraph_t g = nullptr;
+    PSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      f();
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(stream, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    PSX_HIP(hipStreamEndCapture(stream, &g));
+    hipGraphExec_t ge_ = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw Error(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return ge_;
+  }
+
+  void run_cached(std::map<int64_t, hipGraphExec_t>& cache, int64_t ks, void (SimImpl::*fn)(int64_t)) {
+    if (!tuning.use_graph) {
+      (this->*fn)(ks);
+      return;
+    }
+    auto it = cache.find(ks);
+    if (it == cache.end()) it = cache.emplace(ks, capture_graph([&] { (this->*fn)(ks); })).first;
+    PSX_HIP(hipGraphLaunch(it->second, stream));
+  }
+
   void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0, double* gparams) override {
     if (win_nseg <= 0) throw Error(PSX_ERR_STATE, "psx_window_backward needs a psx_window_forward first");
     const size_t nn = (size_t)n;
-    const size_t nc = (size_t)ncells + 1;
     PSX_HIP(hipMemcpyAsync(lxA.p, dlx, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
     PSX_HIP(hipMemcpyAsync(lvA.p, dlv, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
     for (DevBuf* b : {&gk, &ge, &gg}) PSX_HIP(launch_fill<T>(stream, (int64_t)nn, b->as<T>(), T(0)));
     int nb;
-    const BlockInfo* bt = table(256, &nb);
+    table(256, &nb);
     std::vector<hipEvent_t> ev((size_t)(3 * win_nseg), nullptr);
     for (auto& e : ev) PSX_HIP(hipEventCreate(&e));
     struct EvGuard {
@@ -583,30 +666,12 @@ struct SimImpl final : SimBase {
       }
     } guard{ev};
     for (int64_t sgm = win_nseg - 1; sgm >= 0; --sgm) {
-      restore_state(sgm);
       const int64_t ks = std::min(win_every, win_steps - sgm * win_every);
+      load_state(sgm, seg_pos.p, seg_vel.p, seg_id.p);
       PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm)], stream));
-      for (int64_t k = 0; k < ks; ++k) step_once(nullptr, k);
+      run_cached(g_replay, ks, &SimImpl::replay_segment);
       PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm + 1)], stream));
-      for (int64_t k = ks - 1; k >= 0; --k) {
-        Vec4<T>* sp = rec_s_pos.as<Vec4<T>>() + k * nn;
-        Vec4<T>* sv = rec_s_vel.as<Vec4<T>>() + k * nn;
-        const int32_t* si = rec_s_id.as<int32_t>() + k * nn;
-        // home-cell bucketing of the snapshot (hash only: nothing is written back to sp/sv)
-        bucket(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
-               offH.as<uint32_t>(), nullptr, 0, nullptr);
-        PSX_HIP(launch_adj_pairs<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
-                                    rec_s_off.as<uint32_t>() + k * nc, rec_s_cell.as<int32_t>() + k * nn,
-                                    posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
-                                    offH.as<uint32_t>(), lxA.as<T>(), lvA.as<T>(), lxB.as<T>(),
-                                    lvB.as<T>(), gk.as<T>(), ge.as<T>()));
-        PSX_HIP(launch_adj_container<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(),
-                                        rec_pre_pos.as<Vec4<T>>() + k * nn,
-                                        rec_pre_vel.as<Vec4<T>>() + k * nn,
-                                        rec_pre_id.as<int32_t>() + k * nn, tgeo_d.as<TriGeo<T>>(),
-                                        tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
-                                        lxB.as<T>(), lvB.as<T>(), lxA.as<T>(), lvA.as<T>(), gg.as<T>()));
-      }
+      run_cached(g_adjoint, ks, &SimImpl::adjoint_segment);
       PSX_HIP(hipEventRecord(ev[(size_t)(3 * sgm + 2)], stream));
     }
     PSX_HIP(hipMemcpyAsync(dlx0, lxA.p, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
@@ -628,14 +693,17 @@ struct SimImpl final : SimBase {
       win_replay_ms += a;
       win_adjoint_ms += b;
     }
-    restore_state(win_nseg);  // back to the window's final state
+    // back to the window's final state
+    load_state(win_nseg, pos[0].p, vel[0].p, id[0].p);
+    cur_pos = 0;
+    cur_vel = 0;
+    PSX_HIP(hipStreamSynchronize(stream));
   }
 
   void window_stats(double* replay_ms, double* adjoint_ms) override {
     if (replay_ms) *replay_ms = win_replay_ms;
     if (adjoint_ms) *adjoint_ms = win_adjoint_ms;
   }
-  double win_replay_ms = 0.0, win_adjoint_ms = 0.0;
 
   void set_params(int32_t scene, const psx_params& p) override {
     SceneHost& sh = scenes[(size_t)scene];

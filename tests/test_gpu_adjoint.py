@@ -19,8 +19,10 @@ def _weights(n, seed=3):
     return rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
 
 
-def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10):
+def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10, graph=False):
     sim = psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
+    if graph:
+        sim.set_tuning(use_graph=1)
     dt = torch.float64 if sc.params.dtype == "f64" else torch.float32
     x0 = torch.tensor(sc.pos, dtype=dt, device="cuda", requires_grad=True)
     v0 = torch.tensor(sc.vel, dtype=dt, device="cuda", requires_grad=True)
@@ -32,7 +34,12 @@ def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10):
         (vN.double() * torch.tensor(wv, device="cuda")).sum()
     loss.backward()
     out = {"x0": x0.grad.double().cpu().numpy(), "v0": v0.grad.double().cpu().numpy(),
-           "k": float(k.grad), "es": float(es.grad), "gz": float(gz.grad), "loss": float(loss)}
+           "k": float(k.grad), "es": float(es.grad), "gz": float(gz.grad),
+           "loss": float(loss.detach())}
+    # after the backward the handle holds the window's final state again
+    p, v = sim.state()
+    assert np.array_equal(p, xN.detach().double().cpu().numpy())
+    assert np.array_equal(v, vN.detach().double().cpu().numpy())
     sim.close()
     return out
 
@@ -79,13 +86,16 @@ def test_adjoint_matches_torch_autograd(psx_mod, mesh):
         assert _close(h[key], t[key], 1e-6), (key, h[key], t[key])
 
 
-def test_adjoint_checkpoint_spacing_invariant(psx_mod):
+def test_adjoint_checkpoint_spacing_and_graph_invariant(psx_mod):
+    """Checkpoint spacing and graph replay change neither the recomputed states nor the sums."""
     sc = adjoint_scene(True)
     wx, wv = _weights(sc.radius.shape[0])
     a = _hip(psx_mod, sc, wx, wv, every=7)
     b = _hip(psx_mod, sc, wx, wv, every=50)
+    c = _hip(psx_mod, sc, wx, wv, every=10, graph=True)
     for key in ("x0", "v0", "k", "es", "gz"):
         assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), key
+        assert np.array_equal(np.asarray(a[key]), np.asarray(c[key])), key
 
 
 @pytest.mark.parametrize("mesh", [False, True])
