@@ -41,6 +41,18 @@ def algorithmic_bytes(n: int, cells: int, s: int) -> dict:
     }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the plain-copy calibration), or None."""
+    f = REPO / "profiles" / f"pmc_{kernel}_{workload}.txt"
+    if not f.exists():
+        return None
+    for row in f.read_text().splitlines():
+        if row.startswith("traffic_bytes_per_launch:"):
+            return float(row.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(sc, steps: int) -> dict:
     from oracle import ref
     threads = min(16, os.cpu_count() or 1)
@@ -197,6 +209,9 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hopper, batched and adjoint measurements")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="finish with one plain device-to-device copy of this many MiB (the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -243,7 +258,7 @@ def main() -> int:
     achieved = algo[dom] / (dom_ms * 1e-3)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                "traffic": None, "avg_launch_ms": round(dom_ms, 5),
+                "traffic": measured_traffic(args.workload, dom), "avg_launch_ms": round(dom_ms, 5),
                 "algo_bytes_per_launch": algo[dom], "profiled_steps": psteps}
     sim.close()
 
@@ -276,6 +291,15 @@ def main() -> int:
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+    if args.copy_calibration_mb:
+        import torch
+        nbytes = args.copy_calibration_mb << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench.py: copy calibration, {nbytes} bytes read and written", file=sys.stderr)
     return 0
 
 

@@ -17,9 +17,30 @@
 // XCDs round-robin, so workgroup b is given tile (b % 8) * ceil(tiles / 8) + b / 8, and each XCD
 // (with its own L2) sweeps one contiguous eighth of the cell-sorted particles, where neighbour
 // columns are close in memory.
+#include <hip/hip_ext.h>
+
 #include "psx_kernels.hpp"
 
 namespace psx {
+
+// Timing events for the next launch (set by the profiler): the launch is then issued with
+// hipExtLaunchKernelGGL, whose start/stop events take the dispatch's own begin/end timestamps.
+static thread_local hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+  g_ev_start = start;
+  g_ev_stop = stop;
+}
+
+#define PSX_LAUNCH(kern, grid, block, shm, st, ...)                                           \
+  do {                                                                                       \
+    if (g_ev_start) {                                                                        \
+      hipExtLaunchKernelGGL(kern, grid, block, shm, st, g_ev_start, g_ev_stop, 0, __VA_ARGS__); \
+      g_ev_start = g_ev_stop = nullptr;                                                      \
+    } else {                                                                                 \
+      hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                           \
+    }                                                                                        \
+  } while (0)
 
 // ---------------------------------------------------------------------------------------------
 // wave64 and launch helpers
@@ -542,7 +563,7 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of, uint32_t* slot,
                             uint32_t* count, const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode) {
-  hipLaunchKernelGGL(k_integrate<T>, dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
+  PSX_LAUNCH((k_integrate<T>), dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
                      pos, vel, pos_o, vel_o, cell_of, slot, count, tgeo, tbox, tcell_off, flags,
                      mode);
   return hipGetLastError();
@@ -550,19 +571,19 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
 
 hipError_t launch_scan_reduce(hipStream_t st, const uint32_t* count, uint32_t* tile_sums,
                               int32_t ntiles) {
-  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, count, tile_sums);
+  PSX_LAUNCH(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, count, tile_sums);
   return hipGetLastError();
 }
 
 hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, uint32_t* tile_pre,
                              int32_t ntiles) {
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, tile_pre, ntiles);
+  PSX_LAUNCH(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, tile_pre, ntiles);
   return hipGetLastError();
 }
 
 hipError_t launch_scan_down(hipStream_t st, uint32_t* count, uint32_t* offset,
                             const uint32_t* tile_pre, int32_t ntiles) {
-  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_pre,
+  PSX_LAUNCH(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_pre,
                      ntiles);
   return hipGetLastError();
 }
@@ -571,7 +592,7 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
                               const uint32_t* slot, const uint32_t* offset, const int32_t* id,
                               int32_t* tmp) {
   const int32_t tiles = blocks_for(n, block);
-  hipLaunchKernelGGL(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
+  PSX_LAUNCH(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
                      slot, offset, id, tmp);
   return hipGetLastError();
 }
@@ -582,7 +603,7 @@ hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_
                               const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
                               Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o) {
   const int32_t tiles = blocks_for(n, block);
-  hipLaunchKernelGGL(k_rank_gather<T>, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles,
+  PSX_LAUNCH((k_rank_gather<T>), dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles,
                      cell_of, offset, tmp, pos, vel, id, pos_o, vel_o, id_o, cell_o);
   return hipGetLastError();
 }
@@ -591,7 +612,7 @@ template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
                         const uint32_t* offset, Vec4<T>* vout) {
-  hipLaunchKernelGGL(k_pairs<T>, dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes, pos,
+  PSX_LAUNCH((k_pairs<T>), dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes, pos,
                      vsn, offset, vout);
   return hipGetLastError();
 }

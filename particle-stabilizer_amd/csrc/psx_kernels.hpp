@@ -8,6 +8,9 @@ namespace psx {
 enum : int { kModeIntegrate = 1, kModeContainer = 2, kModeCheck = 4 };
 enum : uint32_t { kFlagNonFinite = 1u };
 
+// Events that time the next kernel launch from its dispatch timestamps (consumed by the launch).
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile

@@ -431,11 +431,13 @@ struct SimImpl final : SimBase {
   }
 
   // One kernel launch, optionally bracketed by timing events.
+  // One kernel launch; when profiling, its own dispatch timestamps go to the profiler's events.
   template <class F>
   void launch(int k, Profiler* prof, F&& f) {
-    if (prof) PSX_HIP(hipEventRecord(prof->begin(k), stream));
-    PSX_HIP(f());
-    if (prof) PSX_HIP(hipEventRecord(prof->end(k), stream));
+    if (prof) set_launch_events(prof->begin(k), prof->end(k));
+    const hipError_t e = f();
+    set_launch_events(nullptr, nullptr);
+    PSX_HIP(e);
   }
 
   // Bucketing: k_integrate in `mode` reads (ps, vs) and, when it integrates or applies the
