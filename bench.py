@@ -32,9 +32,7 @@ def algorithmic_bytes(n: int, cells: int, s: int) -> dict:
     """Bytes each kernel must move per launch (each array touched once; s = scalar bytes)."""
     return {
         "k_integrate": n * (13 * s + 4),
-        "k_scan_reduce": 4 * cells,
-        "k_scan_tiles": 0,
-        "k_scan_down": 12 * cells,
+        "k_scan": 12 * cells,
         "k_scatter_ids": 16 * n,
         "k_rank_gather": n * (16 * s + 12),
         "k_pairs": 11 * s * n + 8 * cells,
@@ -82,7 +80,8 @@ def bench_hopper(steps: int = 100, warmup: int = 10) -> dict:
     sc = scenes.hopper_settle()
     n = sc.radius.shape[0]
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
-    sim.set_tuning(use_graph=1)
+    from psx.tune import autotune
+    tuning = autotune(sim)["tuning"]
     sim.step(warmup)
     p0, v0 = sim.state()
     dt = _timed(sim, steps)
@@ -97,7 +96,7 @@ def bench_hopper(steps: int = 100, warmup: int = 10) -> dict:
     return {"workload": f"hopper_settle: {n:,} spheres, 8,192-triangle mesh, f32",
             "value": round(n * steps / dt, 1), "unit": "particle-steps/s",
             "ms_per_step": round(1e3 * dt / steps, 4), "steps": steps, "kernels": kern,
-            "k_pairs_frac_hbm": round(algo["k_pairs"] / (pm * 1e-3) / HBM_PEAK, 4)}
+            "tuning": tuning, "k_pairs_frac_hbm": round(algo["k_pairs"] / (pm * 1e-3) / HBM_PEAK, 4)}
 
 
 def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
@@ -108,7 +107,8 @@ def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
     scs = scenes.batched_boxes()
     total = sum(sc.radius.shape[0] for sc in scs)
     b = psx.Batch(scs)
-    b.set_tuning(use_graph=1)
+    from psx.tune import autotune
+    tuning = autotune(b)["tuning"]
     b.step(warmup)
     tb = _timed(b, steps)
     cells = b.num_cells
@@ -116,6 +116,7 @@ def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
     sims = [psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) for sc in scs]
     import ctypes
     for s in sims:
+        s.set_tuning(**tuning)
         s.step(warmup)
     for s in sims:
         s.synchronize()
@@ -152,7 +153,7 @@ def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
             "per_scene_one_stream": round(total * steps / t_one, 1),
             "per_scene_stream_each": round(total * steps / t_many, 1),
             "speedup_vs_one_stream": round(t_one / tb, 2),
-            "speedup_vs_stream_each": round(t_many / tb, 2)}
+            "speedup_vs_stream_each": round(t_many / tb, 2), "tuning": tuning}
 
 
 def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
@@ -165,7 +166,8 @@ def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
     sc = scenes.calibration_scene()
     n = sc.radius.shape[0]
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
-    sim.set_tuning(use_graph=1)
+    from psx.tune import autotune
+    autotune(sim)
     dev = torch.device("cuda")
     gx = torch.randn((n, 3), device=dev, dtype=torch.float32)
     gv = torch.randn((n, 3), device=dev, dtype=torch.float32)
@@ -207,6 +209,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes instead of autotuning them")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hopper, batched and adjoint measurements")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -231,6 +235,10 @@ def main() -> int:
 
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
     sim.set_tuning(use_graph=0 if args.no_graph else 1)
+    if not args.no_tune:
+        from psx.tune import autotune
+        autotune(sim, use_graph=not args.no_graph)
+    tuning = sim.tuning
     sim.step(args.warmup)
     sim.synchronize()
     p0, v0 = sim.state()
@@ -270,7 +278,7 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {n:,} spheres"
                                + (f", {sc.mesh.shape[0]:,}-triangle mesh container" if sc.mesh is not None else ", box container")
                                + f", {dtype}", "n_particles": n, "cells": cells,
-                   "graph": not args.no_graph},
+                   "graph": not args.no_graph, "tuning": tuning},
         "roofline": roofline,
         "kernels": kernels,
     }

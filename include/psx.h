@@ -77,7 +77,7 @@ typedef struct psx_tuning {
   int32_t block_sort;       /* workgroup size of the scatter and rank/gather kernels */
   int32_t block_pairs;      /* workgroup size of the pair-contact kernel */
   int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
-  int32_t pair_variant;     /* pair-contact kernel variant; 0 (one lane per particle) */
+  int32_t pair_variant;     /* lanes per particle in the pair-contact kernel: 0 or 1, 2, 4, 8, 16 */
 } psx_tuning;
 
 typedef struct psx_sim psx_sim;

@@ -4,9 +4,8 @@
 // container, pairs) is these launches on one stream:
 //   k_integrate    x += v*dt; v += g*dt; cell hash of the new x; box walls or mesh contact;
 //                  per-cell count (atomic, returns the particle's arrival slot)
-//   k_scan_reduce  \
-//   k_scan_tiles    > exclusive scan of the per-cell counts into offsets (tile sums, tile
-//   k_scan_down    /  prefixes, each tile rescanned with wave64 scans); clears the counts
+//   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
+//                  look-back over tiles, wave64 scans inside a tile); clears the counts
 //   k_scatter_ids  ids to offset[cell] + arrival slot (an unordered cell list)
 //   k_rank_gather  rank of each particle among its cell mates by id (stable order) and gather of
 //                  the state into cell-sorted order
@@ -276,61 +275,44 @@ __global__ void __launch_bounds__(1024)
 }
 
 // ---------------------------------------------------------------------------------------------
-// K3: exclusive scan of the per-cell counts over tiles of kScanTile cells: the tile sums, one
-// workgroup turning them into tile prefixes, then every tile rescanned (16 cells per thread summed
-// serially, wave64 inclusive scans of the thread totals, LDS wave totals, the tile prefix added);
-// offsets written, counts cleared for the next step.
+// K3: exclusive scan of the per-cell counts in one pass (decoupled look-back). Each workgroup
+// scans one tile of kScanTile cells: 16 cells per thread summed serially, wave64 inclusive scans
+// of the thread totals, LDS wave totals. It publishes the tile total, looks back over the
+// published totals of the tiles before it (64 at a time, one per lane of wave 0) until it meets a
+// published inclusive prefix, publishes its own inclusive prefix, writes the offsets and clears
+// the counts for the next step.
+//
+// Forward progress: tiles are handed out in the order workgroups start (a ticket counter), so a
+// workgroup only waits for tiles held by workgroups that started before it and never wait for a
+// later one; tile 0 waits for nobody. Status words carry the launch's epoch (ticket / ntiles + 1),
+// so the words left by the previous launch are never mistaken for current ones and nothing has to
+// be cleared between launches. Every wait is bounded: past kScanSpinLimit polls the predecessor is
+// taken as a prefix of 0, the stall flag is raised for the host, and all offsets stay at or below
+// their true values (so every index derived from them stays inside its array).
 // ---------------------------------------------------------------------------------------------
+constexpr uint64_t kTileAggregate = 1, kTilePrefix = 2;
+constexpr uint32_t kEpochMask = (1u << 30) - 1;
+constexpr int kScanSpinLimit = 1 << 16;
+
+__device__ inline uint64_t tile_word(uint32_t value, uint64_t status, uint32_t epoch) {
+  return (uint64_t)value | (status << 32) | ((uint64_t)epoch << 34);
+}
+
 __global__ void __launch_bounds__(kScanThreads)
-    k_scan_reduce(const uint32_t* __restrict__ count, uint32_t* __restrict__ tile_sums) {
+    k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
+           uint64_t* __restrict__ tile_state, unsigned long long* __restrict__ ticket,
+           int32_t ntiles, uint32_t* __restrict__ flags) {
   __shared__ uint32_t wsum[kScanThreads / 64];
-  const uint4* __restrict__ src =
-      reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile);
-  uint32_t s = 0;
-#pragma unroll
-  for (int q = 0; q < kScanPerThread / 4; ++q) {  // coalesced: consecutive lanes, consecutive uint4
-    const uint4 u = src[q * kScanThreads + threadIdx.x];
-    s += (u.x + u.y) + (u.z + u.w);
-  }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-  if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
-  __syncthreads();
+  __shared__ uint32_t s_tile, s_epoch, s_prefix;
   if (threadIdx.x == 0) {
-    uint32_t t = 0;
-#pragma unroll
-    for (int w = 0; w < kScanThreads / 64; ++w) t += wsum[w];
-    tile_sums[blockIdx.x] = t;
+    const unsigned long long t = atomicAdd(ticket, 1ull);
+    s_tile = (uint32_t)(t % (unsigned long long)ntiles);
+    s_epoch = (uint32_t)((t / (unsigned long long)ntiles + 1ull) & kEpochMask);
   }
-}
-
-__global__ void __launch_bounds__(1024)
-    k_scan_tiles(uint32_t* __restrict__ tile_sums, uint32_t* __restrict__ tile_pre, int32_t ntiles) {
-  __shared__ uint32_t wsum[16];
-  const int32_t per = (ntiles + 1023) / 1024;
-  const int32_t b = (int32_t)threadIdx.x * per;
-  const int32_t e = min(b + per, ntiles);
-  uint32_t s = 0;
-  for (int32_t i = b; i < e; ++i) s += tile_sums[i];
-  const uint32_t wi = wave_inclusive_scan(s);
-  const int w = threadIdx.x >> 6;
-  if (lane_id() == 63) wsum[w] = wi;
   __syncthreads();
-  uint32_t run = wi - s;
-  for (int q = 0; q < w; ++q) run += wsum[q];
-  for (int32_t i = b; i < e; ++i) {
-    const uint32_t c = tile_sums[i];
-    tile_pre[i] = run;
-    run += c;
-  }
-  if (threadIdx.x == 1023) tile_pre[ntiles] = run;
-}
-
-__global__ void __launch_bounds__(kScanThreads)
-    k_scan_down(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
-                const uint32_t* __restrict__ tile_pre, int32_t ntiles) {
-  __shared__ uint32_t wsum[kScanThreads / 64];
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPerThread;
+  const int32_t tile = (int32_t)s_tile;
+  const uint32_t epoch = s_epoch;
+  const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanPerThread;
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + base);
   uint32_t v[kScanPerThread];
 #pragma unroll
@@ -346,9 +328,54 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int k = 0; k < kScanPerThread; ++k) tot += v[k];
   const uint32_t wi = wave_inclusive_scan(tot);
   const int w = threadIdx.x >> 6;
-  if (lane_id() == 63) wsum[w] = wi;
+  const int lane = lane_id();
+  if (lane == 63) wsum[w] = wi;
   __syncthreads();
-  uint32_t run = tile_pre[blockIdx.x] + wi - tot;
+  uint32_t agg = 0;
+#pragma unroll
+  for (int q = 0; q < kScanThreads / 64; ++q) agg += wsum[q];
+
+  if (w == 0) {
+    if (lane == 0)
+      __hip_atomic_store(tile_state + tile,
+                         tile_word(agg, tile == 0 ? kTilePrefix : kTileAggregate, epoch),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t prefix = 0;
+    int32_t end = tile;  // look back over tiles [end - 64, end)
+    bool done = (tile == 0);
+    while (!done) {
+      const int32_t j = end - 1 - lane;
+      uint64_t word = tile_word(0, kTilePrefix, epoch);  // lanes before tile 0 end the walk
+      if (j >= 0) {
+        for (int spins = 0;; ++spins) {
+          word = __hip_atomic_load(tile_state + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((uint32_t)(word >> 34) == epoch && ((word >> 32) & 3u) != 0) break;
+          if (spins >= kScanSpinLimit) {
+            atomicOr(flags, kFlagScanStall);
+            word = tile_word(0, kTilePrefix, epoch);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      const uint64_t pm = __ballot(((word >> 32) & 3u) == kTilePrefix);
+      const int stop = pm ? __ffsll((unsigned long long)pm) - 1 : 63;  // nearest prefix
+      uint32_t val = (lane <= stop) ? (uint32_t)word : 0u;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) val += __shfl_xor(val, d, 64);
+      prefix += val;
+      done = pm != 0;
+      end -= 64;
+    }
+    if (lane == 0) {
+      if (tile > 0)
+        __hip_atomic_store(tile_state + tile, tile_word(prefix + agg, kTilePrefix, epoch),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_prefix = prefix;
+    }
+  }
+  __syncthreads();
+  uint32_t run = s_prefix + wi - tot;
   for (int q = 0; q < w; ++q) run += wsum[q];
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
@@ -365,7 +392,8 @@ __global__ void __launch_bounds__(kScanThreads)
     reinterpret_cast<uint4*>(offset + base)[q] = o;
     src[q] = zero;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) offset[(size_t)ntiles * kScanTile] = tile_pre[ntiles];
+  if (tile == ntiles - 1 && threadIdx.x == kScanThreads - 1)
+    offset[(size_t)ntiles * kScanTile] = run;  // total: the offset one past the last cell
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -420,21 +448,21 @@ struct PairSelf {
   T ks, es;
 };
 
-// One candidate j of particle i. The prefilter (fused multiply-adds, conservative by the margin)
-// only skips candidates the exact test would reject; the test and the kick are evaluated in the
-// reference's expression order.
+// One candidate j of particle i: its kick on i, or false when they do not touch. The prefilter
+// (fused multiply-adds, conservative by the margin) only skips candidates the exact test would
+// reject; the test and the kick are evaluated in the reference's expression order.
 template <class T>
-__device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4<T>& xj,
-                                               const Vec4<T>* __restrict__ vsn, uint32_t q,
-                                               T& ax, T& ay, T& az) {
+__device__ __forceinline__ bool pair_kick(const PairSelf<T>& me, const Vec4<T>& xj,
+                                          const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
+                                          T& fy, T& fz) {
   // isHit: distance(x_i, x_j) = length(x_j - x_i) < r_i + r_j
   const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
-  if (fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr) return;
+  if (fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr) return false;
   const T dd = dot3(ex, ey, ez, ex, ey, ez);
   const T R = me.x.w + xj.w;
-  if (dd > (R * R) * PairMargin<T>::value) return;
+  if (dd > (R * R) * PairMargin<T>::value) return false;
   const T len = sqrt(dd);
-  if (!(len < R)) return;
+  if (!(len < R)) return false;
   const Vec4<T> vj = vsn[q];
   // d = x_i - x_j; dot(d, d) and length(d) equal dd and len bit for bit.
   const T dx = -ex, dy = -ey, dz = -ez;
@@ -443,9 +471,54 @@ __device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4
                (dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz) / dd);
   const T inv = T(1) / len;
   const T kov = me.ks * ov;
-  ax = ax + ((-((s3 * dx) * me.es)) + kov * (dx * inv));
-  ay = ay + ((-((s3 * dy) * me.es)) + kov * (dy * inv));
-  az = az + ((-((s3 * dz) * me.es)) + kov * (dz * inv));
+  fx = (-((s3 * dx) * me.es)) + kov * (dx * inv);
+  fy = (-((s3 * dy) * me.es)) + kov * (dy * inv);
+  fz = (-((s3 * dz) * me.es)) + kov * (dz * inv);
+  return true;
+}
+
+template <class T>
+__device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4<T>& xj,
+                                               const Vec4<T>* __restrict__ vsn, uint32_t q,
+                                               T& ax, T& ay, T& az) {
+  T fx, fy, fz;
+  if (pair_kick(me, xj, vsn, q, fx, fy, fz)) {
+    ax = ax + fx;
+    ay = ay + fy;
+    az = az + fz;
+  }
+}
+
+// Particle i's constants and the nine column slices [lo[d], hi[d]) of its 27 neighbour cells.
+template <class T>
+__device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
+                                           const Vec4<T>* __restrict__ pos,
+                                           const Vec4<T>* __restrict__ vsn,
+                                           const uint32_t* __restrict__ offset, int32_t p,
+                                           PairSelf<T>& me, uint32_t (&lo)[9], uint32_t (&hi)[9]) {
+  const GridDev& g = s->grid;
+  me.x = pos[p];
+  me.v = vsn[p];
+  const T rr = me.x.w + s->rmax;
+  me.thr = (rr * rr) * PairMargin<T>::value;
+  me.ks = s->spring;
+  me.es = s->es;
+  // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped)
+  const int32_t hx = axis_index((double)me.x.x, g.min[0], g.h, g.n[0]);
+  const int32_t hy = axis_index((double)me.x.y, g.min[1], g.h, g.n[1]);
+  const int32_t hz = axis_index((double)me.x.z, g.min[2], g.h, g.n[2]);
+  const int32_t zlo = max(hz - 1, 0), zhi = min(hz + 1, g.n[2] - 1);
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const int32_t bx = hx - 1 + d / 3, by = hy - 1 + d % 3;
+    lo[d] = 0;
+    hi[d] = 0;
+    if (bx >= 0 && bx < g.n[0] && by >= 0 && by < g.n[1]) {
+      const int32_t cb = g.cell_base + (bx * g.n[1] + by) * g.n[2];
+      lo[d] = offset[cb + zlo];
+      hi[d] = offset[cb + zhi + 1];
+    }
+  }
 }
 
 template <class T>
@@ -459,34 +532,9 @@ __global__ void __launch_bounds__(1024)
   const BlockInfo bi = btab[tb];
   const int32_t p = bi.p_start + (int32_t)threadIdx.x;
   if (p >= bi.p_end) return;
-  const SceneDev<T>* __restrict__ s = scenes + bi.scene;
-  const GridDev& g = s->grid;
   PairSelf<T> me;
-  me.x = pos[p];
-  me.v = vsn[p];
-  {
-    const T rr = me.x.w + s->rmax;
-    me.thr = (rr * rr) * PairMargin<T>::value;
-    me.ks = s->spring;
-    me.es = s->es;
-  }
-  // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped)
-  const int32_t hx = axis_index((double)me.x.x, g.min[0], g.h, g.n[0]);
-  const int32_t hy = axis_index((double)me.x.y, g.min[1], g.h, g.n[1]);
-  const int32_t hz = axis_index((double)me.x.z, g.min[2], g.h, g.n[2]);
-  const int32_t zlo = max(hz - 1, 0), zhi = min(hz + 1, g.n[2] - 1);
   uint32_t lo[9], hi[9];
-#pragma unroll
-  for (int d = 0; d < 9; ++d) {
-    const int32_t bx = hx - 1 + d / 3, by = hy - 1 + d % 3;
-    lo[d] = 0;
-    hi[d] = 0;
-    if (bx >= 0 && bx < g.n[0] && by >= 0 && by < g.n[1]) {
-      const int32_t cb = g.cell_base + (bx * g.n[1] + by) * g.n[2];
-      lo[d] = offset[cb + zlo];
-      hi[d] = offset[cb + zhi + 1];
-    }
-  }
+  pair_setup(scenes + bi.scene, pos, vsn, offset, p, me, lo, hi);
   T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
@@ -502,6 +550,54 @@ __global__ void __launch_bounds__(1024)
     }
   }
   vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+}
+
+// The same contact sum with G lanes per particle (G divides 64, so a group never straddles a
+// wave). Each column slice is walked in rounds of G consecutive candidates, lane g of the group
+// testing candidate (round start + g); the kicks found in a round are then added by every lane of
+// the group in ascending candidate order, fetched from their lanes with shuffles. Every lane of a
+// group thus performs the single-lane kernel's additions in the same order and ends with the same
+// bit pattern; lane 0 of the group writes it. More lanes per particle shorten the serial chain of
+// dependent loads per lane (about 81 candidates / G rounds), which is what bounds the kernel when
+// the particles alone are too few to fill the GPU.
+template <class T, int G>
+__global__ void __launch_bounds__(1024)
+    k_pairs_lanes(int32_t ntab, const BlockInfo* __restrict__ btab,
+                  const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ pos,
+                  const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
+                  Vec4<T>* __restrict__ vout) {
+  static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "lanes per particle: 2, 4, 8 or 16");
+  const int32_t tb = xcd_tile(blockIdx.x, ntab);
+  if (tb >= ntab) return;
+  const BlockInfo bi = btab[tb];
+  const int32_t g = (int32_t)threadIdx.x % G;
+  const int32_t p = bi.p_start + (int32_t)threadIdx.x / G;
+  if (p >= bi.p_end) return;  // whole groups leave together
+  PairSelf<T> me;
+  uint32_t lo[9], hi[9];
+  pair_setup(scenes + bi.scene, pos, vsn, offset, p, me, lo, hi);
+  const int gbase = lane_id() & ~(G - 1);
+  T ax = me.v.x, ay = me.v.y, az = me.v.z;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const uint32_t e = hi[d];
+    for (uint32_t q0 = lo[d]; q0 < e; q0 += G) {  // the same trip count on every lane of a group
+      const uint32_t q = q0 + (uint32_t)g;
+      T fx = T(0), fy = T(0), fz = T(0);
+      bool hit = false;
+      if (q < e && (int32_t)q != p) hit = pair_kick(me, pos[q], vsn, q, fx, fy, fz);
+      uint32_t mine = (uint32_t)(__ballot(hit) >> gbase) & ((1u << G) - 1u);
+      while (mine) {  // identical on every lane of the group
+        const int src = gbase + __ffs(mine) - 1;
+        mine &= mine - 1u;
+        const T kx = __shfl(fx, src, 64), ky = __shfl(fy, src, 64), kz = __shfl(fz, src, 64);
+        ax = ax + kx;
+        ay = ay + ky;
+        az = az + kz;
+      }
+    }
+  }
+  if (g == 0) vout[p] = Vec4<T>{ax, ay, az, me.v.w};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -569,22 +665,10 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
   return hipGetLastError();
 }
 
-hipError_t launch_scan_reduce(hipStream_t st, const uint32_t* count, uint32_t* tile_sums,
-                              int32_t ntiles) {
-  PSX_LAUNCH(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, count, tile_sums);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, uint32_t* tile_pre,
-                             int32_t ntiles) {
-  PSX_LAUNCH(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, tile_pre, ntiles);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_down(hipStream_t st, uint32_t* count, uint32_t* offset,
-                            const uint32_t* tile_pre, int32_t ntiles) {
-  PSX_LAUNCH(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_pre,
-                     ntiles);
+hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
+                       unsigned long long* ticket, int32_t ntiles, uint32_t* flags) {
+  PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ticket,
+             ntiles, flags);
   return hipGetLastError();
 }
 
@@ -609,11 +693,29 @@ hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_
 }
 
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
                         const uint32_t* offset, Vec4<T>* vout) {
-  PSX_LAUNCH((k_pairs<T>), dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes, pos,
-                     vsn, offset, vout);
+  const dim3 grid(xcd_grid(ntab)), blk(block);
+  switch (lanes) {
+    case 1:
+      PSX_LAUNCH((k_pairs<T>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      break;
+    case 2:
+      PSX_LAUNCH((k_pairs_lanes<T, 2>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      break;
+    case 4:
+      PSX_LAUNCH((k_pairs_lanes<T, 4>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      break;
+    case 8:
+      PSX_LAUNCH((k_pairs_lanes<T, 8>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      break;
+    case 16:
+      PSX_LAUNCH((k_pairs_lanes<T, 16>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -643,7 +745,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                             const uint32_t*, const int32_t*, const Vec4<T>*,   \
                                             const Vec4<T>*, const int32_t*, Vec4<T>*, Vec4<T>*,\
                                             int32_t*, int32_t*);                               \
-  template hipError_t launch_pairs<T>(hipStream_t, int, int, const BlockInfo*,                 \
+  template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
                                       const uint32_t*, Vec4<T>*);                              \
   template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,    \

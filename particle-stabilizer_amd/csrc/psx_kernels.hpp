@@ -6,7 +6,7 @@
 namespace psx {
 
 enum : int { kModeIntegrate = 1, kModeContainer = 2, kModeCheck = 4 };
-enum : uint32_t { kFlagNonFinite = 1u };
+enum : uint32_t { kFlagNonFinite = 1u, kFlagScanStall = 2u };
 
 // Events that time the next kernel launch from its dispatch timestamps (consumed by the launch).
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
@@ -23,12 +23,11 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode);
 
-hipError_t launch_scan_reduce(hipStream_t st, const uint32_t* count, uint32_t* tile_sums,
-                              int32_t ntiles);
-hipError_t launch_scan_tiles(hipStream_t st, uint32_t* tile_sums, uint32_t* tile_pre,
-                             int32_t ntiles);
-hipError_t launch_scan_down(hipStream_t st, uint32_t* count, uint32_t* offset,
-                            const uint32_t* tile_pre, int32_t ntiles);
+// One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
+// (the last entry is the total); clears the counts. tile_state (ntiles words) and ticket persist
+// between launches and are zeroed once at allocation; launches on one handle must be serialized.
+hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
+                       unsigned long long* ticket, int32_t ntiles, uint32_t* flags);
 
 hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
                               const uint32_t* slot, const uint32_t* offset, const int32_t* id,
@@ -40,8 +39,9 @@ hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_
                               const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
                               Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o);
 
+// lanes: lanes per particle (1, 2, 4, 8 or 16); btab holds block / lanes particles per workgroup.
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
                         const uint32_t* offset, Vec4<T>* vout);
 

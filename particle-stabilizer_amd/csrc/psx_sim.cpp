@@ -224,10 +224,8 @@ struct DevBuf {
   }
 };
 
-static const char* kKernelNames[kNumKernels] = {"k_integrate",   "k_scan_reduce",
-                                                "k_scan_tiles",  "k_scan_down",
-                                                "k_scatter_ids", "k_rank_gather",
-                                                "k_pairs"};
+static const char* kKernelNames[kNumKernels] = {"k_integrate", "k_scan", "k_scatter_ids",
+                                                "k_rank_gather", "k_pairs"};
 
 // ---------------------------------------------------------------------------------------------
 // the simulation
@@ -238,7 +236,7 @@ struct SimImpl final : SimBase {
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
   DevBuf cell_of, slot, tmp;
-  DevBuf count, offset, tile_sums, tile_pre, flags;
+  DevBuf count, offset, tile_state, ticket, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tcell_off_d;
   DevBuf io;  // 6n T staging for state transfer
   std::map<int, std::unique_ptr<DevBuf>> btab;
@@ -398,8 +396,8 @@ struct SimImpl final : SimBase {
     tmp.alloc((size_t)nn * 4);
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1) * 4);
-    tile_sums.alloc(((size_t)ntiles + 1) * 4);
-    tile_pre.alloc(((size_t)ntiles + 1) * 4);
+    tile_state.alloc((size_t)ntiles * 8);
+    ticket.alloc(8);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
     scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
@@ -412,8 +410,8 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpy(id[0].p, ids.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
     PSX_HIP(hipMemset(count.p, 0, count.bytes));
     PSX_HIP(hipMemset(offset.p, 0, offset.bytes));
-    PSX_HIP(hipMemset(tile_sums.p, 0, tile_sums.bytes));
-    PSX_HIP(hipMemset(tile_pre.p, 0, tile_pre.bytes));
+    PSX_HIP(hipMemset(tile_state.p, 0, tile_state.bytes));
+    PSX_HIP(hipMemset(ticket.p, 0, ticket.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
     PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
     if (!tgeo.empty())
@@ -426,7 +424,7 @@ struct SimImpl final : SimBase {
     tcell_entries = (int64_t)tbox.size();
     int nb;
     table(tuning.block_integrate, &nb);
-    table(tuning.block_pairs, &nb);
+    table(pair_particles_per_block(), &nb);
     PSX_HIP(hipDeviceSynchronize());
   }
 
@@ -459,19 +457,14 @@ struct SimImpl final : SimBase {
                                  flags.as<uint32_t>(), mode);
     });
     launch(1, prof, [&] {
-      return launch_scan_reduce(stream, count.as<uint32_t>(), tile_sums.as<uint32_t>(), ntiles);
+      return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(),
+                         ticket.as<unsigned long long>(), ntiles, flags.as<uint32_t>());
     });
     launch(2, prof, [&] {
-      return launch_scan_tiles(stream, tile_sums.as<uint32_t>(), tile_pre.as<uint32_t>(), ntiles);
-    });
-    launch(3, prof, [&] {
-      return launch_scan_down(stream, count.as<uint32_t>(), off, tile_pre.as<uint32_t>(), ntiles);
-    });
-    launch(4, prof, [&] {
       return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
                                 slot.as<uint32_t>(), off, is, tmp.as<int32_t>());
     });
-    launch(5, prof, [&] {
+    launch(3, prof, [&] {
       return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(), off,
                                    tmp.as<int32_t>(), writes ? po : ps, writes ? vo : vs, is, pd,
                                    vd, idd, cell_sorted);
@@ -486,13 +479,17 @@ struct SimImpl final : SimBase {
            id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr, mode, prof);
   }
 
+  // Lanes per particle of the pair kernel (tuning.pair_variant; 0 means 1).
+  int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
+  int pair_particles_per_block() const { return tuning.block_pairs / pair_lanes(); }
+
   void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
              Profiler* prof) {
     int nb_p;
-    const BlockInfo* bt_p = table(tuning.block_pairs, &nb_p);
-    launch(6, prof, [&] {
-      return launch_pairs<T>(stream, nb_p, tuning.block_pairs, bt_p, scenes_d.as<SceneDev<T>>(),
-                             ps, vs, off, vout);
+    const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
+    launch(4, prof, [&] {
+      return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(), bt_p,
+                             scenes_d.as<SceneDev<T>>(), ps, vs, off, vout);
     });
   }
 
@@ -558,7 +555,7 @@ struct SimImpl final : SimBase {
       rs_vnew.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_id.alloc((size_t)every * nn * 4);
       rs_cell.alloc((size_t)every * nn * 4);
-      rs_off.alloc((size_t)every * ((size_t)cpad + 1) * 4);  // k_scan_down writes cpad + 1
+      rs_off.alloc((size_t)every * ((size_t)cpad + 1) * 4);  // k_scan writes cpad + 1
       for (DevBuf* b : {&lxA, &lvA, &lxB, &lvB}) b->alloc(3 * nn * sizeof(T));
       for (DevBuf* b : {&gk, &ge, &gg}) b->alloc(nn * sizeof(T));
       gout.alloc(3 * sizeof(double) * scenes.size());
@@ -797,7 +794,7 @@ struct SimImpl final : SimBase {
     tuning = t;
     int nb;
     table(tuning.block_integrate, &nb);
-    table(tuning.block_pairs, &nb);
+    table(pair_particles_per_block(), &nb);
   }
 
   void get_state(double* hp, double* hv) override {
@@ -1062,6 +1059,8 @@ int psx_synchronize(psx_sim* sim) {
   if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
   PSX_TRY({
     const uint32_t f = sim->impl->read_flags();
+    if (f & psx::kFlagScanStall)
+      return set_error(PSX_ERR_HIP, "cell-offset scan: a tile prefix was not published in time");
     if (f & psx::kFlagNonFinite)
       return set_error(PSX_ERR_NUMERIC, "non-finite particle position or velocity produced by a step");
     return PSX_OK;
@@ -1087,7 +1086,9 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
     if (b < 64 || b > 1024 || (b % 64) != 0)
       return set_error(PSX_ERR_INVALID, "block sizes must be multiples of 64 in [64, 1024]");
   if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
-  if (c.pair_variant != 0) return set_error(PSX_ERR_INVALID, "pair_variant must be 0");
+  if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
+      c.pair_variant != 8 && c.pair_variant != 16)
+    return set_error(PSX_ERR_INVALID, "pair_variant (lanes per particle) must be 0, 1, 2, 4, 8 or 16");
   PSX_TRY({
     sim->impl->retune(c);
     return PSX_OK;

@@ -53,9 +53,13 @@ def test_bucketing_edge_positions(psx_mod, oracle, dtype):
         _bucket_compare(oracle, s, pos, dtype, h_r, prm.box_min, prm.box_max)
 
 
-VARIANTS = [dict(block_pairs=256, block_integrate=256), dict(block_pairs=128, block_integrate=64),
-            dict(block_pairs=64, block_integrate=1024, block_sort=64),
-            dict(block_pairs=1024, block_integrate=128, block_sort=1024)]
+# Launch shapes: workgroup sizes and lanes per particle in the pair kernel (pair_variant). Every
+# shape must reproduce the oracle bit for bit.
+VARIANTS = [dict(block_pairs=256, block_integrate=256),
+            dict(block_pairs=128, block_integrate=64, pair_variant=2),
+            dict(block_pairs=64, block_integrate=1024, block_sort=64, pair_variant=4),
+            dict(block_pairs=1024, block_integrate=128, block_sort=1024, pair_variant=8),
+            dict(block_pairs=512, pair_variant=16)]
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -98,6 +102,28 @@ def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
     p, v = hip_run(sc, steps)
     assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
     assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16])
+def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
+    """Multi-lane pair kernel on the mesh scene, eager and graph-replayed."""
+    sc = small_cylinder(dtype=dtype)
+    ref_p, ref_v = oracle_run(oracle, sc, 40)
+    for graph in (False, True):
+        p, v = hip_run(sc, 40, graph=graph, pair_variant=lanes, block_pairs=256)
+        assert_bitwise(as_dtype(p, dtype), ref_p, f"pos lanes={lanes} graph={graph}")
+        assert_bitwise(as_dtype(v, dtype), ref_v, f"vel lanes={lanes} graph={graph}")
+
+
+def test_tuning_rejects_bad_lanes(psx_mod):
+    sc = dense_box(dtype="f64")
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        for bad in (3, 32, -1):
+            with pytest.raises(psx_mod.PsxError):
+                s.set_tuning(pair_variant=bad)
+        s.set_tuning(pair_variant=4)
+        assert s.tuning["pair_variant"] == 4
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -148,3 +174,19 @@ def test_nonfinite_flag(psx_mod):
         with pytest.raises(psx_mod.PsxError) as e:
             s.synchronize()
         assert e.value.code == 5
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_autotuned_run_bitexact(psx_mod, oracle, dtype, tmp_path):
+    """The autotuner's sweep leaves the state as it was and its chosen shape reproduces the oracle."""
+    from psx.tune import autotune
+    sc = small_cylinder(dtype=dtype)
+    ref_p, ref_v = oracle_run(oracle, sc, 30)
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
+        out = autotune(s, steps=2, rounds_per_shape=1, cache=tmp_path / "t.json")
+        assert s.tuning["pair_variant"] == out["tuning"]["pair_variant"]
+        s.step(30)
+        s.synchronize()
+        p, v = s.state()
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")

@@ -1,0 +1,89 @@
+"""Host logic of the autotuner (psx/tune.py) against a stand-in handle: the fastest shape of each
+kernel is chosen, the state is restored, and the JSON cache is written and honoured."""
+import json
+
+import numpy as np
+import pytest
+
+from psx import tune
+
+
+class FakeSim:
+    """Duck-typed handle whose kernel times are a known function of the launch shape."""
+
+    def __init__(self):
+        self.n = 1000
+        self.dtype = np.float32
+        self.num_cells = 4096
+        self.num_scenes = 1
+        self.tuning = {"block_integrate": 256, "block_sort": 256, "block_pairs": 256,
+                       "use_graph": 0, "pair_variant": 0}
+        self.pos = np.arange(3000.0).reshape(1000, 3)
+        self.vel = -self.pos
+        self.profiled = 0
+        self.restored = None
+
+    def set_tuning(self, **kw):
+        self.tuning.update(kw)
+
+    def state(self):
+        return self.pos.copy(), self.vel.copy()
+
+    def set_state(self, p, v):
+        self.restored = (p, v)
+
+    def profile(self, steps):
+        self.profiled += 1
+        t = self.tuning
+        g = max(1, t["pair_variant"])
+        return {
+            "k_integrate": {"ms": 0.01 + abs(t["block_integrate"] - 512) * 1e-5},
+            "k_scan": {"ms": 0.004},
+            "k_scatter_ids": {"ms": 0.004 + abs(t["block_sort"] - 128) * 1e-6},
+            "k_rank_gather": {"ms": 0.005 + abs(t["block_sort"] - 128) * 1e-6},
+            "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6},
+        }
+
+
+def test_picks_fastest_shapes_and_restores_state():
+    sim = FakeSim()
+    out = tune.autotune(sim, steps=2)
+    assert out["tuning"] == {"block_integrate": 512, "block_sort": 128, "block_pairs": 256,
+                             "pair_variant": 4, "use_graph": 1}
+    assert sim.tuning["pair_variant"] == 4 and sim.tuning["use_graph"] == 1
+    assert sim.restored is not None
+    np.testing.assert_array_equal(sim.restored[0], sim.pos)
+    np.testing.assert_array_equal(sim.restored[1], sim.vel)
+    # every pair shape and every block size was timed
+    assert sim.profiled == len(tune.PAIR_SHAPES) * 2
+
+
+def test_cache_written_and_reused(tmp_path):
+    cache = tmp_path / "tune.json"
+    sim = FakeSim()
+    first = tune.autotune(sim, steps=1, cache=cache)
+    assert not first["cached"]
+    data = json.loads(cache.read_text())
+    assert tune.signature(sim) in data
+    assert "float32" in tune.signature(sim)
+    sim2 = FakeSim()
+    again = tune.autotune(sim2, cache=cache)
+    assert again["cached"] and sim2.profiled == 0
+    assert again["tuning"] == first["tuning"]
+    assert sim2.tuning["pair_variant"] == first["tuning"]["pair_variant"]
+    forced = tune.autotune(FakeSim(), steps=1, cache=cache, force=True)
+    assert not forced["cached"]
+
+
+def test_signature_separates_workloads():
+    a, b = FakeSim(), FakeSim()
+    b.num_cells = 8192
+    assert tune.signature(a) != tune.signature(b)
+
+
+@pytest.mark.parametrize("bad", ["{not json", ""])
+def test_unreadable_cache_is_ignored(tmp_path, bad):
+    cache = tmp_path / "tune.json"
+    cache.write_text(bad)
+    out = tune.autotune(FakeSim(), steps=1, cache=cache)
+    assert not out["cached"]
