@@ -31,10 +31,9 @@ METRIC = "particle-steps/s on 1 GPU (plain vs batched); adjoint/forward step tim
 def algorithmic_bytes(n: int, cells: int, s: int) -> dict:
     """Bytes each kernel must move per launch (each array touched once; s = scalar bytes)."""
     return {
-        "k_integrate": n * (13 * s + 4),
+        "k_integrate": n * (13 * s + 16),
         "k_scan": 12 * cells,
-        "k_scatter_ids": 16 * n,
-        "k_rank_gather": n * (16 * s + 12),
+        "k_rank_gather": n * (16 * s + 24),
         "k_pairs": 11 * s * n + 8 * cells,
     }
 

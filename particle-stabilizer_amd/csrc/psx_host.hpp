@@ -12,7 +12,7 @@
 
 namespace psx {
 
-constexpr int kNumKernels = 5;
+constexpr int kNumKernels = 4;
 
 int set_error(int code, const std::string& msg);
 

@@ -3,10 +3,11 @@
 // One step (PhysicsEngine.cpp:38-48, CPU order: integrate, bucket from the pre-container positions,
 // container, pairs) is these launches on one stream:
 //   k_integrate    x += v*dt; v += g*dt; cell hash of the new x; box walls or mesh contact;
-//                  per-cell count (atomic, returns the particle's arrival slot)
+//                  per-cell count (atomic, returns the particle's arrival slot), and the particle's
+//                  id written to that slot of its cell's id block (kCellIds slots; arrivals past
+//                  them go to an overflow list)
 //   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
 //                  look-back over tiles, wave64 scans inside a tile); clears the counts
-//   k_scatter_ids  ids to offset[cell] + arrival slot (an unordered cell list)
 //   k_rank_gather  rank of each particle among its cell mates by id (stable order) and gather of
 //                  the state into cell-sorted order
 //   k_pairs        27-cell spring/restitution velocity kick, Jacobi over the sorted snapshot
@@ -220,9 +221,10 @@ template <class T>
 __global__ void __launch_bounds__(1024)
     k_integrate(int32_t ntab, const BlockInfo* __restrict__ btab,
                 const SceneDev<T>* __restrict__ scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* __restrict__ cell_of,
-                uint32_t* __restrict__ slot, uint32_t* __restrict__ count,
-                const TriGeo<T>* __restrict__ tgeo,
+                const int32_t* __restrict__ id, Vec4<T>* pos_o, Vec4<T>* vel_o,
+                int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
+                int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
+                uint32_t* __restrict__ ovf_ctr, const TriGeo<T>* __restrict__ tgeo,
                 const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
                 uint32_t* __restrict__ flags, int mode) {
   const int32_t tb = xcd_tile(blockIdx.x, ntab);
@@ -270,7 +272,14 @@ __global__ void __launch_bounds__(1024)
       vel_o[p] = v;
     }
     cell_of[p] = c;
-    slot[p] = atomicAdd(count + c, 1u);
+    const int32_t me = id[p];
+    const uint32_t slot = atomicAdd(count + c, 1u);
+    if (slot < (uint32_t)kCellIds) {
+      cell_ids[(size_t)c * kCellIds + slot] = me;
+    } else {  // at most n entries in all
+      const uint32_t o = atomicAdd(ovf_ctr, 1u);
+      ovf[o] = make_int2(c, me);
+    }
   }
 }
 
@@ -301,7 +310,7 @@ __device__ inline uint64_t tile_word(uint32_t value, uint64_t status, uint32_t e
 __global__ void __launch_bounds__(kScanThreads)
     k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
            uint64_t* __restrict__ tile_state, unsigned long long* __restrict__ ticket,
-           int32_t ntiles, uint32_t* __restrict__ flags) {
+           int32_t ntiles, uint32_t* __restrict__ ovf_ctr, uint32_t* __restrict__ flags) {
   __shared__ uint32_t wsum[kScanThreads / 64];
   __shared__ uint32_t s_tile, s_epoch, s_prefix;
   if (threadIdx.x == 0) {
@@ -312,6 +321,12 @@ __global__ void __launch_bounds__(kScanThreads)
   __syncthreads();
   const int32_t tile = (int32_t)s_tile;
   const uint32_t epoch = s_epoch;
+  if (tile == 0 && threadIdx.x == 0) {
+    // the overflow list of the cell id blocks is complete (written by the previous kernel):
+    // publish its length for the rank/gather kernel and restart the counter for the next pass
+    ovf_ctr[1] = ovf_ctr[0];
+    ovf_ctr[0] = 0u;
+  }
   const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanPerThread;
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + base);
   uint32_t v[kScanPerThread];
@@ -397,37 +412,48 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 // ---------------------------------------------------------------------------------------------
-// K4: unordered cell lists, then stable rank + gather
+// K4: stable rank + gather. The reference appends ids to a cell in ascending order
+// (ParticleModel.cpp:55-61), so a particle's sorted slot is offset[cell] + the number of cell mates
+// with a smaller id. The mates are read from the cell's id block (one 32-byte load of kCellIds
+// slots, the first count of which are this pass's) and, for a cell with more than kCellIds
+// members, from the overflow list.
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-    k_scatter_ids(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
-                  const uint32_t* __restrict__ slot, const uint32_t* __restrict__ offset,
-                  const int32_t* __restrict__ id, int32_t* __restrict__ tmp) {
-  const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
-  if (p >= n) return;
-  tmp[offset[cell_of[p]] + slot[p]] = id[p];
-}
-
-// Stable order inside a cell (the reference appends ids in ascending order,
-// ParticleModel.cpp:55-61): rank = number of cell mates with a smaller id.
 template <class T>
 __global__ void __launch_bounds__(1024)
     k_rank_gather(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
-                  const uint32_t* __restrict__ offset, const int32_t* __restrict__ tmp,
+                  const uint32_t* __restrict__ offset, const int32_t* __restrict__ cell_ids,
+                  const int2* __restrict__ ovf, const uint32_t* __restrict__ ovf_ctr,
                   const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
                   const int32_t* __restrict__ id, Vec4<T>* __restrict__ pos_o,
                   Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
                   int32_t* __restrict__ cell_o) {
+  static_assert(kCellIds == 8, "the id block is read as two int4");
   const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
   if (p >= n) return;
   const int32_t c = cell_of[p];
-  const uint32_t b = offset[c], e = offset[c + 1];
   const int32_t me = id[p];
+  const Vec4<T> x = pos[p];
+  const Vec4<T> v = vel[p];
+  const uint32_t b = offset[c], e = offset[c + 1];
+  const uint32_t cnt = e - b;
+  const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
+  const int4 m0 = blk[0], m1 = blk[1];
+  const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
   uint32_t rank = 0;
-  for (uint32_t k = b; k < e; ++k) rank += (tmp[k] < me) ? 1u : 0u;
-  const uint32_t dst = b + rank;
-  pos_o[dst] = pos[p];
-  vel_o[dst] = vel[p];
+#pragma unroll
+  for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
+  if (cnt > (uint32_t)kCellIds) {
+    const uint32_t len = ovf_ctr[1];
+    for (uint32_t o = 0; o < len; ++o) {
+      const int2 q = ovf[o];
+      rank += (q.x == c && q.y < me) ? 1u : 0u;
+    }
+  }
+  // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
+  // arrays even if a stalled scan (reported through the flags) left them short
+  const uint32_t dst = min(b + rank, (uint32_t)(n - 1));
+  pos_o[dst] = x;
+  vel_o[dst] = v;
   id_o[dst] = me;
   if (cell_o) cell_o[dst] = c;
 }
@@ -448,16 +474,22 @@ struct PairSelf {
   T ks, es;
 };
 
-// One candidate j of particle i: its kick on i, or false when they do not touch. The prefilter
-// (fused multiply-adds, conservative by the margin) only skips candidates the exact test would
-// reject; the test and the kick are evaluated in the reference's expression order.
+// The prefilter of candidate j (fused multiply-adds, conservative by the margin): false only for
+// candidates the exact test would reject. NaN coordinates pass it and fail the exact test.
 template <class T>
-__device__ __forceinline__ bool pair_kick(const PairSelf<T>& me, const Vec4<T>& xj,
-                                          const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
-                                          T& fy, T& fz) {
+__device__ __forceinline__ bool pair_near(const PairSelf<T>& me, const Vec4<T>& xj) {
+  const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
+  return !(fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr);
+}
+
+// The exact contact test and kick of candidate j on particle i, in the reference's expression
+// order; false when they do not touch.
+template <class T>
+__device__ __forceinline__ bool pair_exact(const PairSelf<T>& me, const Vec4<T>& xj,
+                                           const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
+                                           T& fy, T& fz) {
   // isHit: distance(x_i, x_j) = length(x_j - x_i) < r_i + r_j
   const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
-  if (fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr) return false;
   const T dd = dot3(ex, ey, ez, ex, ey, ez);
   const T R = me.x.w + xj.w;
   if (dd > (R * R) * PairMargin<T>::value) return false;
@@ -475,6 +507,14 @@ __device__ __forceinline__ bool pair_kick(const PairSelf<T>& me, const Vec4<T>& 
   fy = (-((s3 * dy) * me.es)) + kov * (dy * inv);
   fz = (-((s3 * dz) * me.es)) + kov * (dz * inv);
   return true;
+}
+
+// One candidate j of particle i: its kick on i, or false when they do not touch.
+template <class T>
+__device__ __forceinline__ bool pair_kick(const PairSelf<T>& me, const Vec4<T>& xj,
+                                          const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
+                                          T& fy, T& fz) {
+  return pair_near(me, xj) && pair_exact(me, xj, vsn, q, fx, fy, fz);
 }
 
 template <class T>
@@ -521,6 +561,45 @@ __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
   }
 }
 
+// Candidates are taken four at a time: one address, four independent loads, four prefilters and a
+// single branch into the rare path, which applies the contacts of the chunk in ascending order.
+// The loads may run up to three entries past the end of a slice; the sorted position arrays carry
+// kPairPad spare entries so that they stay inside the allocation, and those entries are masked out.
+template <class T>
+__device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                           const Vec4<T>* __restrict__ vsn, uint32_t q, uint32_t e,
+                                           uint32_t self, T& ax, T& ay, T& az) {
+  const Vec4<T>* __restrict__ src = pos + q;  // one address; the other three are offsets
+  const Vec4<T> c0 = src[0], c1 = src[1], c2 = src[2], c3 = src[3];
+  const bool m0 = q != self && pair_near(me, c0);
+  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
+  const bool m2 = q + 2 < e && q + 2 != self && pair_near(me, c2);
+  const bool m3 = q + 3 < e && q + 3 != self && pair_near(me, c3);
+  if (m0 || m1 || m2 || m3) {
+    T fx, fy, fz;
+    if (m0 && pair_exact(me, c0, vsn, q, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+    if (m1 && pair_exact(me, c1, vsn, q + 1, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+    if (m2 && pair_exact(me, c2, vsn, q + 2, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+    if (m3 && pair_exact(me, c3, vsn, q + 3, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024)
     k_pairs(int32_t ntab, const BlockInfo* __restrict__ btab,
@@ -538,16 +617,8 @@ __global__ void __launch_bounds__(1024)
   T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
-    uint32_t q = lo[d];
     const uint32_t e = hi[d];
-    if (q < e) {
-      Vec4<T> nxt = pos[q];  // one candidate in flight ahead of the one being tested
-      for (; q < e; ++q) {
-        const Vec4<T> cur = nxt;
-        if (q + 1 < e) nxt = pos[q + 1];
-        if ((int32_t)q != p) pair_candidate(me, cur, vsn, q, ax, ay, az);
-      }
-    }
+    for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
   }
   vout[p] = Vec4<T>{ax, ay, az, me.v.w};
 }
@@ -656,39 +727,33 @@ static inline int32_t blocks_for(int64_t n, int b) { return (int32_t)((n + b - 1
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                            Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of, uint32_t* slot,
-                            uint32_t* count, const TriGeo<T>* tgeo, const TriBox<T>* tbox,
+                            const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of,
+                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
+                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode) {
   PSX_LAUNCH((k_integrate<T>), dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
-                     pos, vel, pos_o, vel_o, cell_of, slot, count, tgeo, tbox, tcell_off, flags,
-                     mode);
+             pos, vel, id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, tgeo, tbox,
+             tcell_off, flags, mode);
   return hipGetLastError();
 }
 
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       unsigned long long* ticket, int32_t ntiles, uint32_t* flags) {
+                       unsigned long long* ticket, int32_t ntiles, uint32_t* ovf_ctr,
+                       uint32_t* flags) {
   PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ticket,
-             ntiles, flags);
-  return hipGetLastError();
-}
-
-hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* slot, const uint32_t* offset, const int32_t* id,
-                              int32_t* tmp) {
-  const int32_t tiles = blocks_for(n, block);
-  PSX_LAUNCH(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
-                     slot, offset, id, tmp);
+             ntiles, ovf_ctr, flags);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* offset, const int32_t* tmp, const Vec4<T>* pos,
-                              const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
-                              Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o) {
+                              const uint32_t* offset, const int32_t* cell_ids, const int2* ovf,
+                              const uint32_t* ovf_ctr, const Vec4<T>* pos, const Vec4<T>* vel,
+                              const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+                              int32_t* cell_o) {
   const int32_t tiles = blocks_for(n, block);
-  PSX_LAUNCH((k_rank_gather<T>), dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles,
-                     cell_of, offset, tmp, pos, vel, id, pos_o, vel_o, id_o, cell_o);
+  PSX_LAUNCH((k_rank_gather<T>), dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
+             offset, cell_ids, ovf, ovf_ctr, pos, vel, id, pos_o, vel_o, id_o, cell_o);
   return hipGetLastError();
 }
 
@@ -738,13 +803,15 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
 #define PSX_INSTANTIATE(T)                                                                      \
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
                                           const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,  \
-                                          Vec4<T>*, Vec4<T>*, int32_t*,                        \
-                                          uint32_t*, uint32_t*, const TriGeo<T>*,              \
-                                          const TriBox<T>*, const int32_t*, uint32_t*, int);   \
+                                          const int32_t*, Vec4<T>*, Vec4<T>*, int32_t*,        \
+                                          uint32_t*, int32_t*, int2*, uint32_t*,               \
+                                          const TriGeo<T>*, const TriBox<T>*, const int32_t*,  \
+                                          uint32_t*, int);                                     \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,          \
-                                            const uint32_t*, const int32_t*, const Vec4<T>*,   \
-                                            const Vec4<T>*, const int32_t*, Vec4<T>*, Vec4<T>*,\
-                                            int32_t*, int32_t*);                               \
+                                            const uint32_t*, const int32_t*, const int2*,      \
+                                            const uint32_t*, const Vec4<T>*, const Vec4<T>*,   \
+                                            const int32_t*, Vec4<T>*, Vec4<T>*, int32_t*,      \
+                                            int32_t*);                                         \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
                                       const uint32_t*, Vec4<T>*);                              \

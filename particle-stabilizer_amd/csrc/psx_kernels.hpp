@@ -14,30 +14,35 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
+constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the overflow list
+// Spare entries after every sorted position array the pair kernel reads (its chunked loads may
+// reach three entries past the last particle).
+constexpr int kPairPad = 4;
 
+// cell_ids: kCellIds slots per cell (C_pad * kCellIds); ovf: n entries (cell, id); ovf_ctr[0]
+// counts the overflow entries of this pass and must be 0 on entry (k_scan resets it).
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                            Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of, uint32_t* slot,
-                            uint32_t* count,
+                            const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of,
+                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
                             const TriGeo<T>* tgeo, const TriBox<T>* tbox,
                             const int32_t* tcell_off, uint32_t* flags, int mode);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
-// (the last entry is the total); clears the counts. tile_state (ntiles words) and ticket persist
-// between launches and are zeroed once at allocation; launches on one handle must be serialized.
+// (the last entry is the total); clears the counts, moves ovf_ctr[0] to ovf_ctr[1] and zeroes
+// ovf_ctr[0]. tile_state (ntiles words) and ticket persist between launches and are zeroed once at
+// allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       unsigned long long* ticket, int32_t ntiles, uint32_t* flags);
-
-hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* slot, const uint32_t* offset, const int32_t* id,
-                              int32_t* tmp);
+                       unsigned long long* ticket, int32_t ntiles, uint32_t* ovf_ctr,
+                       uint32_t* flags);
 
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* offset, const int32_t* tmp, const Vec4<T>* pos,
-                              const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o,
-                              Vec4<T>* vel_o, int32_t* id_o, int32_t* cell_o);
+                              const uint32_t* offset, const int32_t* cell_ids, const int2* ovf,
+                              const uint32_t* ovf_ctr, const Vec4<T>* pos, const Vec4<T>* vel,
+                              const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+                              int32_t* cell_o);
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16); btab holds block / lanes particles per workgroup.
 template <class T>

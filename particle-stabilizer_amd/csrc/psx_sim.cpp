@@ -224,8 +224,8 @@ struct DevBuf {
   }
 };
 
-static const char* kKernelNames[kNumKernels] = {"k_integrate", "k_scan", "k_scatter_ids",
-                                                "k_rank_gather", "k_pairs"};
+static const char* kKernelNames[kNumKernels] = {"k_integrate", "k_scan", "k_rank_gather",
+                                                "k_pairs"};
 
 // ---------------------------------------------------------------------------------------------
 // the simulation
@@ -235,7 +235,7 @@ struct SimImpl final : SimBase {
   // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
-  DevBuf cell_of, slot, tmp;
+  DevBuf cell_of, cell_ids, ovf, ovf_ctr;
   DevBuf count, offset, tile_state, ticket, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tcell_off_d;
   DevBuf io;  // 6n T staging for state transfer
@@ -387,13 +387,14 @@ struct SimImpl final : SimBase {
 
     // device allocations
     for (int k = 0; k < 2; ++k) {
-      pos[k].alloc((size_t)nn * sizeof(Vec4<T>));
+      pos[k].alloc(((size_t)nn + kPairPad) * sizeof(Vec4<T>));
       vel[k].alloc((size_t)nn * sizeof(Vec4<T>));
       id[k].alloc((size_t)nn * sizeof(int32_t));
     }
     cell_of.alloc((size_t)nn * 4);
-    slot.alloc((size_t)nn * 4);
-    tmp.alloc((size_t)nn * 4);
+    cell_ids.alloc((size_t)cpad * kCellIds * 4);
+    ovf.alloc((size_t)nn * 8);
+    ovf_ctr.alloc(8);
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1) * 4);
     tile_state.alloc((size_t)ntiles * 8);
@@ -411,6 +412,8 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemset(count.p, 0, count.bytes));
     PSX_HIP(hipMemset(offset.p, 0, offset.bytes));
     PSX_HIP(hipMemset(tile_state.p, 0, tile_state.bytes));
+    PSX_HIP(hipMemset(ovf_ctr.p, 0, ovf_ctr.bytes));
+    PSX_HIP(hipMemset(cell_ids.p, 0, cell_ids.bytes));
     PSX_HIP(hipMemset(ticket.p, 0, ticket.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
     PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
@@ -439,8 +442,8 @@ struct SimImpl final : SimBase {
   }
 
   // Bucketing: k_integrate in `mode` reads (ps, vs) and, when it integrates or applies the
-  // container, writes the result to (po, vo) (which may alias ps, vs); scan into `off`; stable
-  // scatter of the ids `is`; gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
+  // container, writes the result to (po, vo) (which may alias ps, vs) and files the ids `is` into
+  // the cell id blocks; scan into `off`; stable rank and gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
   // cell_sorted when it is not NULL.
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, Vec4<T>* po, Vec4<T>* vo, const int32_t* is,
               Vec4<T>* pd, Vec4<T>* vd, int32_t* idd, uint32_t* off, int32_t* cell_sorted,
@@ -451,23 +454,22 @@ struct SimImpl final : SimBase {
     const bool writes = (mode & (kModeIntegrate | kModeContainer)) != 0;
     launch(0, prof, [&] {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
-                                 scenes_d.as<SceneDev<T>>(), ps, vs, po, vo, cell_of.as<int32_t>(),
-                                 slot.as<uint32_t>(), count.as<uint32_t>(), tgeo_d.as<TriGeo<T>>(),
-                                 tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
-                                 flags.as<uint32_t>(), mode);
+                                 scenes_d.as<SceneDev<T>>(), ps, vs, is, po, vo,
+                                 cell_of.as<int32_t>(), count.as<uint32_t>(),
+                                 cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
+                                 tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
+                                 tcell_off_d.as<int32_t>(), flags.as<uint32_t>(), mode);
     });
     launch(1, prof, [&] {
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(),
-                         ticket.as<unsigned long long>(), ntiles, flags.as<uint32_t>());
+                         ticket.as<unsigned long long>(), ntiles, ovf_ctr.as<uint32_t>(),
+                         flags.as<uint32_t>());
     });
     launch(2, prof, [&] {
-      return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
-                                slot.as<uint32_t>(), off, is, tmp.as<int32_t>());
-    });
-    launch(3, prof, [&] {
       return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(), off,
-                                   tmp.as<int32_t>(), writes ? po : ps, writes ? vo : vs, is, pd,
-                                   vd, idd, cell_sorted);
+                                   cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
+                                   writes ? po : ps, writes ? vo : vs, is, pd, vd, idd,
+                                   cell_sorted);
     });
   }
 
@@ -487,7 +489,7 @@ struct SimImpl final : SimBase {
              Profiler* prof) {
     int nb_p;
     const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
-    launch(4, prof, [&] {
+    launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(), bt_p,
                              scenes_d.as<SceneDev<T>>(), ps, vs, off, vout);
     });
@@ -550,7 +552,7 @@ struct SimImpl final : SimBase {
       seg_id.alloc(nn * 4);
       idH.alloc(nn * 4);
       offH.alloc(((size_t)cpad + 1) * 4);
-      rs_pos.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_pos.alloc(((size_t)every * nn + kPairPad) * sizeof(Vec4<T>));
       rs_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_vnew.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_id.alloc((size_t)every * nn * 4);

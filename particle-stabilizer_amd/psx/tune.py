@@ -6,7 +6,7 @@ through the shorter lists), profiles a few steps with the kernels' own dispatch 
 each kernel keeps the shape with the smallest median time over its rounds. Tuned knobs:
 
 - block_integrate: workgroup size of the integrate / hash / container kernel;
-- block_sort: workgroup size of the id scatter and the rank / gather kernels;
+- block_sort: workgroup size of the rank / gather kernel;
 - (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel.
 
 The cell size stays what the reference's grid rule fixes (it changes the neighbour search, not
@@ -68,8 +68,7 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
                        use_graph=0)
         prof = sim.profile(steps)
         for name, d in prof.items():
-            shape = {"k_integrate": bi, "k_scatter_ids": bs, "k_rank_gather": bs,
-                     "k_pairs": (bp, g)}.get(name)
+            shape = {"k_integrate": bi, "k_rank_gather": bs, "k_pairs": (bp, g)}.get(name)
             if shape is None:
                 continue
             samples.setdefault((name, shape), []).append(d["ms"])
@@ -77,21 +76,14 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
         t = sorted(ms)[len(ms) // 2]  # median over the rounds that used this shape
         if name not in best or t < best[name][0]:
             best[name] = (t, shape)
-    # the scatter and the rank/gather share block_sort: pick the size with the smallest sum
-    sort_ms = {}
-    for (name, shape), ms in samples.items():
-        if name in ("k_scatter_ids", "k_rank_gather"):
-            sort_ms.setdefault(shape, 0.0)
-            sort_ms[shape] += sorted(ms)[len(ms) // 2]
-    bs_best = min(sort_ms, key=sort_ms.get)
+    bs_best = best["k_rank_gather"][1]
     bp_best, g_best = best["k_pairs"][1]
     tuning = {"block_integrate": int(best["k_integrate"][1]), "block_sort": int(bs_best),
               "block_pairs": int(bp_best), "pair_variant": int(g_best),
               "use_graph": 1 if use_graph else 0}
     sim.set_tuning(**tuning)
     sim.set_state(p0, v0)
-    kernel_ms = {"k_integrate": best["k_integrate"][0], "k_pairs": best["k_pairs"][0],
-                 "sort": sort_ms[bs_best]}
+    kernel_ms = {k: best[k][0] for k in ("k_integrate", "k_rank_gather", "k_pairs")}
     if path is not None:
         data = _load(path)
         data[key] = {"tuning": tuning, "kernel_ms": kernel_ms}

@@ -97,6 +97,6 @@ def test_null_handles_rejected():
     assert lib.psx_destroy(None) == 0
     assert lib.psx_num_particles(None) == -1
     assert lib.psx_kernel_name(0) == b"k_integrate"
-    assert lib.psx_kernel_name(4) == b"k_pairs"
-    assert lib.psx_kernel_name(5) is None
+    assert lib.psx_kernel_name(3) == b"k_pairs"
+    assert lib.psx_kernel_name(4) is None
     assert lib.psx_kernel_name(99) is None

@@ -55,6 +55,41 @@ def test_bucketing_edge_positions(psx_mod, oracle, dtype):
 
 # Launch shapes: workgroup sizes and lanes per particle in the pair kernel (pair_variant). Every
 # shape must reproduce the oracle bit for bit.
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_bucketing_cell_overflow(psx_mod, oracle, dtype):
+    """Cells holding more particles than the per-cell id block (8 slots): one cell with 300, many
+    with 9-12, the rest sparse. The overflow list must give the same stable order; then a few steps
+    through the same path."""
+    from psx.sim import Params, mass_of
+    rng = np.random.default_rng(11)
+    h_r = 0.5
+    prm = Params(box_min=(0.0, 0.0, 0.0), box_max=(12.0, 12.0, 12.0), dtype=dtype, dt=1e-4,
+                 spring=5.0, restitution=0.3, gravity=(0.0, 0.0, -1.0))
+    h = h_r * 2.0 * float(np.float32(1.05))
+    parts = [rng.uniform(2 * h + 0.01, 3 * h - 0.01, size=(300, 3))]  # one cell
+    for k in range(40):
+        corner = rng.integers(4, 10, size=3) * h
+        parts.append(corner + rng.uniform(0.01, h - 0.01, size=(int(rng.integers(9, 13)), 3)))
+    parts.append(rng.uniform(0.5, 11.5, size=(500, 3)))
+    pos = np.concatenate(parts)
+    rng.shuffle(pos)  # ids in no particular spatial order
+    n = pos.shape[0]
+    r = rng.uniform(0.3, h_r, n)
+    r[0] = h_r
+    vel = rng.normal(0.0, 0.1, size=(n, 3))
+    with psx_mod.Simulation(pos, vel, r, None, prm) as s:
+        rmax = float(as_dtype(r, dtype).max())
+        _bucket_compare(oracle, s, pos, dtype, rmax, prm.box_min, prm.box_max)
+        got = s.bucketing()
+        assert got["count"].max() >= 300
+    from psx.sim import Scene
+    sc = Scene(pos, vel, r, prm, mass_of(r))
+    ref_p, ref_v = oracle_run(oracle, sc, 3)
+    p, v = hip_run(sc, 3)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
 VARIANTS = [dict(block_pairs=256, block_integrate=256),
             dict(block_pairs=128, block_integrate=64, pair_variant=2),
             dict(block_pairs=64, block_integrate=1024, block_sort=64, pair_variant=4),

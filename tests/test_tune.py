@@ -39,7 +39,6 @@ class FakeSim:
         return {
             "k_integrate": {"ms": 0.01 + abs(t["block_integrate"] - 512) * 1e-5},
             "k_scan": {"ms": 0.004},
-            "k_scatter_ids": {"ms": 0.004 + abs(t["block_sort"] - 128) * 1e-6},
             "k_rank_gather": {"ms": 0.005 + abs(t["block_sort"] - 128) * 1e-6},
             "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6},
         }
