@@ -73,8 +73,9 @@ typedef struct psx_scene {
 
 /* Launch shapes, chosen by the autotuner (psx/tune.py) or left at 0 for the built-in defaults. */
 typedef struct psx_tuning {
-  int32_t block_integrate;  /* workgroup size of the integrate/hash/container kernel */
-  int32_t block_sort;       /* workgroup size of the scatter and rank/gather kernels */
+  int32_t block_integrate;  /* workgroup size of the integrate / hash / count kernel */
+  int32_t block_sort;       /* workgroup size of the rank / container / gather kernel
+                               (at most 512 for PSX_F64) */
   int32_t block_pairs;      /* workgroup size of the pair-contact kernel */
   int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
   int32_t pair_variant;     /* lanes per particle in the pair-contact kernel: 0 or 1, 2, 4, 8, 16 */

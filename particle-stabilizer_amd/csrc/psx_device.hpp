@@ -67,6 +67,15 @@ struct alignas(sizeof(T) == 4 ? 32 : 64) TriBox {
   int32_t tri;
 };
 
+// One candidate of a grid cell's triangle list as the forward container kernel reads it: the
+// triangle's AABB first (one short load for the reject test), then its geometry in the same or the
+// next cache line (80 bytes in binary32, 144 in binary64).
+template <class T>
+struct alignas(16) TriEntry {
+  T lo[3], hi[3];
+  TriGeo<T> g;
+};
+
 // Work assignment of one workgroup: its scene and slice of particles.
 struct BlockInfo {
   int32_t scene;

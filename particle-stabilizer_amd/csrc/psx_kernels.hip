@@ -2,14 +2,14 @@
 //
 // One step (PhysicsEngine.cpp:38-48, CPU order: integrate, bucket from the pre-container positions,
 // container, pairs) is these launches on one stream:
-//   k_integrate    x += v*dt; v += g*dt; cell hash of the new x; box walls or mesh contact;
-//                  per-cell count (atomic, returns the particle's arrival slot), and the particle's
-//                  id written to that slot of its cell's id block (kCellIds slots; arrivals past
-//                  them go to an overflow list)
+//   k_integrate    x += v*dt; v += g*dt; cell hash of the new (pre-container) x; per-cell count
+//                  (atomic, returns the particle's arrival slot), and the particle's id written to
+//                  that slot of its cell's id block (kCellIds slots; arrivals past them go to an
+//                  overflow list)
 //   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
 //                  look-back over tiles, wave64 scans inside a tile); clears the counts
-//   k_rank_gather  rank of each particle among its cell mates by id (stable order) and gather of
-//                  the state into cell-sorted order
+//   k_rank_gather  rank of each particle among its cell mates by id (stable order); box walls or
+//                  mesh contact on the particle; gather of the result into cell-sorted order
 //   k_pairs        27-cell spring/restitution velocity kick, Jacobi over the sorted snapshot
 // Particle state lives in cell-sorted order between steps; an id array carries original indices.
 //
@@ -18,6 +18,8 @@
 // (with its own L2) sweeps one contiguous eighth of the cell-sorted particles, where neighbour
 // columns are close in memory.
 #include <hip/hip_ext.h>
+
+#include <cstring>
 
 #include "psx_kernels.hpp"
 
@@ -68,6 +70,35 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
+// How a workgroup finds its particles and its scene's constants. A batch of scenes has one table
+// entry per workgroup (scene, particle slice). A single scene needs no table: its constants travel
+// by value in the kernel arguments and a workgroup's slice follows from its index, which takes a
+// dependent load (table entry, then scene record) off the start of every workgroup.
+template <class T>
+struct Locator {
+  const BlockInfo* btab;       // batches: one entry per workgroup
+  const SceneDev<T>* scenes;   // batches: the scene records
+  int32_t ntab;                // workgroups with work
+  int32_t per_block;           // single scene: particles per workgroup
+  SceneDev<T> one;             // single scene: its constants
+};
+
+// Runs body(scene, p0, p1) for the workgroup's scene and particle slice [p0, p1); padding
+// workgroups return. The scene is passed by reference straight from the table or from the kernel
+// arguments, so no pointer has to merge the two address spaces.
+template <class T, bool kTable, class Body>
+__device__ __forceinline__ void with_slice(const Locator<T>& loc, Body&& body) {
+  const int32_t tb = xcd_tile(blockIdx.x, loc.ntab);
+  if (tb >= loc.ntab) return;
+  if constexpr (kTable) {
+    const BlockInfo bi = loc.btab[tb];
+    body(loc.scenes[bi.scene], bi.p_start, bi.p_end);
+  } else {
+    const int32_t p0 = tb * loc.per_block;
+    body(loc.one, p0, min(p0 + loc.per_block, loc.one.p_end));
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // container contact (per particle)
 // ---------------------------------------------------------------------------------------------
@@ -75,7 +106,7 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
 // BoxContainer::resolveCollisions (BoxContainer.cpp:30-65): lo/hi from the incoming position, then
 // xmin, ymin, zmin, xmax, ymax, zmax.
 template <class T>
-__device__ inline void box_walls(const SceneDev<T>* __restrict__ s, Vec4<T>& x, Vec4<T>& v) {
+__device__ __forceinline__ void box_walls(const SceneDev<T>* __restrict__ s, Vec4<T>& x, Vec4<T>& v) {
   const T r = x.w;
   const T lox = x.x - r, loy = x.y - r, loz = x.z - r;
   const T hix = x.x + r, hiy = x.y + r, hiz = x.z + r;
@@ -135,20 +166,23 @@ __device__ __forceinline__ void mesh_hit(MeshAcc<T>& m, T e, T r, T vx, T vy, T 
   ++m.cnt;
 }
 
-// AABB reject (PolygonContainer.cpp:251-259), closest point and contact test of one candidate.
+// AABB reject of one candidate (PolygonContainer.cpp:251-259).
 template <class T>
-__device__ __forceinline__ bool mesh_touch(const TriBox<T>& bx, const TriGeo<T>* __restrict__ tgeo,
-                                           const Vec3<T>& P, T r, TriGeo<T>& t) {
-  if (P.x - r > bx.hi[0] || P.x + r < bx.lo[0] || P.y - r > bx.hi[1] || P.y + r < bx.lo[1] ||
-      P.z - r > bx.hi[2] || P.z + r < bx.lo[2])
-    return false;
-  t = tgeo[bx.tri];
-  const Vec3<T> q = closest_on_triangle(t, P);
+__device__ __forceinline__ bool mesh_aabb(const T (&lo)[3], const T (&hi)[3], const Vec3<T>& P,
+                                          T r) {
+  return !(P.x - r > hi[0] || P.x + r < lo[0] || P.y - r > hi[1] || P.y + r < lo[1] ||
+           P.z - r > hi[2] || P.z + r < lo[2]);
+}
+
+// Closest point on the triangle and the contact test |q - x| < r (PolygonContainer.cpp:260-263).
+template <class T>
+__device__ __forceinline__ bool mesh_contact(const TriGeo<T>& g, const Vec3<T>& P, T r) {
+  const Vec3<T> q = closest_on_triangle(g, P);
   return dist_to(P, q) < r;
 }
 
 template <class T>
-__device__ inline void mesh_finish(const MeshAcc<T>& m, Vec4<T>& x, Vec4<T>& v) {
+__device__ __forceinline__ void mesh_finish(const MeshAcc<T>& m, Vec4<T>& x, Vec4<T>& v) {
   if (m.cnt > 0) {  // PolygonContainer.cpp:280-285
     const T cf = (T)m.cnt;
     x.x = m.y.x;
@@ -162,33 +196,49 @@ __device__ inline void mesh_finish(const MeshAcc<T>& m, Vec4<T>& x, Vec4<T>& v) 
 
 // Lists longer than this are walked by the whole wave: 64 lanes test 64 candidates at once and the
 // hits are then applied in ascending order, one after the other, exactly as a single lane would.
+// Shorter lists are walked by their own lane four candidates at a time: the four AABBs are loaded
+// together (the entry array carries kTriPad spare entries for the last chunk) and the geometry of
+// each candidate that passes is read in candidate order. The contact test uses the particle's
+// original position (PolygonContainer.cpp:260-263), so only the hits' effect is sequential.
 constexpr int32_t kHeavyList = 12;
 
+// k0, k1: the particle's list [k0, k1) in tent (any values when !active).
 template <class T>
-__device__ inline void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool active, int32_t c,
-                                         const TriGeo<T>* __restrict__ tgeo,
-                                         const TriBox<T>* __restrict__ tbox,
-                                         const int32_t* __restrict__ tcell_off, Vec4<T>& x,
-                                         Vec4<T>& v) {
-  int32_t k0 = 0, k1 = 0;
-  if (active) {
-    const int32_t tc = s->tcell_base + (c - s->grid.cell_base);
-    k0 = tcell_off[tc];
-    k1 = tcell_off[tc + 1];
-  }
+__device__ __forceinline__ void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool active,
+                                                  int32_t k0, int32_t k1,
+                                                  const TriEntry<T>* __restrict__ tent,
+                                                  Vec4<T>& x, Vec4<T>& v) {
   const T e = s->ew;
-  const bool heavy = (k1 - k0) > kHeavyList;
+  const bool heavy = active && (k1 - k0) > kHeavyList;
   if (active && !heavy && k1 > k0) {
     const Vec3<T> P = {x.x, x.y, x.z};
+    const T r = x.w;
     MeshAcc<T> m = {P, T(0), T(0), T(0), 0};
-    for (int32_t k = k0; k < k1; ++k) {
-      TriGeo<T> t;
-      if (mesh_touch(tbox[k], tgeo, P, x.w, t))
-        mesh_hit(m, e, x.w, v.x, v.y, v.z, t.n[0], t.n[1], t.n[2], t.a[0], t.a[1], t.a[2]);
+    for (int32_t kb = k0; kb < k1; kb += 4) {
+      const TriEntry<T>* __restrict__ c4 = tent + kb;
+      uint32_t pass = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        T lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = c4[j].lo[a];
+          hi[a] = c4[j].hi[a];
+        }
+        if (kb + j < k1 && mesh_aabb(lo, hi, P, r)) pass |= 1u << j;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (pass & (1u << j)) {
+          const TriGeo<T> g = c4[j].g;
+          if (mesh_contact(g, P, r))
+            mesh_hit(m, e, r, v.x, v.y, v.z, g.n[0], g.n[1], g.n[2], g.a[0], g.a[1], g.a[2]);
+        }
+      }
     }
     mesh_finish(m, x, v);
   }
-  uint64_t hm = __ballot(active && heavy);
+  uint64_t hm = __ballot(heavy);
   while (hm) {
     const int owner = __ffsll((unsigned long long)hm) - 1;
     hm &= hm - 1;
@@ -199,15 +249,33 @@ __device__ inline void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool
     MeshAcc<T> m = {P, T(0), T(0), T(0), 0};
     for (int32_t base = b0; base < b1; base += 64) {
       const int32_t k = base + lane_id();
-      TriGeo<T> t = {};
-      const bool hit = (k < b1) && mesh_touch(tbox[k], tgeo, P, r, t);
+      T n0 = T(0), n1 = T(0), n2 = T(0), a0 = T(0), a1 = T(0), a2 = T(0);
+      bool hit = false;
+      if (k < b1) {  // only the normal and the first corner outlive the test
+        const TriEntry<T>* __restrict__ ep = tent + k;
+        T lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = ep->lo[a];
+          hi[a] = ep->hi[a];
+        }
+        if (mesh_aabb(lo, hi, P, r)) {
+          const TriGeo<T> g = ep->g;
+          hit = mesh_contact(g, P, r);
+          n0 = g.n[0];
+          n1 = g.n[1];
+          n2 = g.n[2];
+          a0 = g.a[0];
+          a1 = g.a[1];
+          a2 = g.a[2];
+        }
+      }
       uint64_t hits = __ballot(hit);
       while (hits) {  // ascending lane = ascending candidate
         const int src = __ffsll((unsigned long long)hits) - 1;
         hits &= hits - 1;
-        mesh_hit(m, e, r, vx, vy, vz, __shfl(t.n[0], src), __shfl(t.n[1], src),
-                 __shfl(t.n[2], src), __shfl(t.a[0], src), __shfl(t.a[1], src),
-                 __shfl(t.a[2], src));
+        mesh_hit(m, e, r, vx, vy, vz, __shfl(n0, src), __shfl(n1, src), __shfl(n2, src),
+                 __shfl(a0, src), __shfl(a1, src), __shfl(a2, src));
       }
     }
     if (lane_id() == owner) mesh_finish(m, x, v);
@@ -217,62 +285,36 @@ __device__ inline void mesh_contact_wave(const SceneDev<T>* __restrict__ s, bool
 // ---------------------------------------------------------------------------------------------
 // K1: integrate + cell hash + container + counts
 // ---------------------------------------------------------------------------------------------
-template <class T>
+template <class T, bool kTable>
 __global__ void __launch_bounds__(1024)
-    k_integrate(int32_t ntab, const BlockInfo* __restrict__ btab,
-                const SceneDev<T>* __restrict__ scenes, const Vec4<T>* pos, const Vec4<T>* vel,
+    k_integrate(const Locator<T> loc, const Vec4<T>* pos, const Vec4<T>* vel,
                 const int32_t* __restrict__ id, Vec4<T>* pos_o, Vec4<T>* vel_o,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
-                uint32_t* __restrict__ ovf_ctr, const TriGeo<T>* __restrict__ tgeo,
-                const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
-                uint32_t* __restrict__ flags, int mode) {
-  const int32_t tb = xcd_tile(blockIdx.x, ntab);
-  if (tb >= ntab) return;
-  const BlockInfo bi = btab[tb];
-  const int32_t p = bi.p_start + (int32_t)threadIdx.x;
-  const bool active = p < bi.p_end;
-  const SceneDev<T>* __restrict__ s = scenes + bi.scene;
-  Vec4<T> x = {}, v = {};
-  int32_t c = 0;
-  if (active) {
-    x = pos[p];
-    v = vel[p];
+                uint32_t* __restrict__ ovf_ctr, int mode) {
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    if (p >= p1) return;
+    Vec4<T> x = pos[p], v = vel[p];
+    const int32_t me = id[p];
     if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25
-      const T dt = s->dt;
+      const T dt = s.dt;
       x.x = x.x + v.x * dt;
       x.y = x.y + v.y * dt;
       x.z = x.z + v.z * dt;
-      v.x = v.x + s->gdt[0];
-      v.y = v.y + s->gdt[1];
-      v.z = v.z + s->gdt[2];
-    }
-    // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container x.
-    const GridDev& g = s->grid;
-    const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-    const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-    const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-    c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
-  }
-  if (mode & kModeContainer) {
-    if (s->mesh < 0) {
-      if (active) box_walls(s, x, v);
-    } else {
-      mesh_contact_wave(s, active, c, tgeo, tbox, tcell_off, x, v);
-    }
-  }
-  if (active) {
-    if (mode & kModeCheck) {
-      const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
-                      isfinite(v.y) && isfinite(v.z);
-      if (!ok) atomicOr(flags, kFlagNonFinite);
-    }
-    if (mode & (kModeIntegrate | kModeContainer)) {
+      v.x = v.x + s.gdt[0];
+      v.y = v.y + s.gdt[1];
+      v.z = v.z + s.gdt[2];
       pos_o[p] = x;
       vel_o[p] = v;
     }
+    // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container x.
+    const GridDev& g = s.grid;
+    const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+    const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+    const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+    const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
     cell_of[p] = c;
-    const int32_t me = id[p];
     const uint32_t slot = atomicAdd(count + c, 1u);
     if (slot < (uint32_t)kCellIds) {
       cell_ids[(size_t)c * kCellIds + slot] = me;
@@ -280,7 +322,7 @@ __global__ void __launch_bounds__(1024)
       const uint32_t o = atomicAdd(ovf_ctr, 1u);
       ovf[o] = make_int2(c, me);
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -418,44 +460,73 @@ __global__ void __launch_bounds__(kScanThreads)
 // slots, the first count of which are this pass's) and, for a cell with more than kCellIds
 // members, from the overflow list.
 // ---------------------------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(1024)
-    k_rank_gather(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
+template <class T, bool kTable>
+__global__ void __launch_bounds__(kMaxBlockSort<T>)
+    k_rank_gather(const Locator<T> loc, const int32_t* __restrict__ cell_of,
                   const uint32_t* __restrict__ offset, const int32_t* __restrict__ cell_ids,
                   const int2* __restrict__ ovf, const uint32_t* __restrict__ ovf_ctr,
                   const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
-                  const int32_t* __restrict__ id, Vec4<T>* __restrict__ pos_o,
-                  Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
-                  int32_t* __restrict__ cell_o) {
+                  const int32_t* __restrict__ id, const TriEntry<T>* __restrict__ tent,
+                  const int32_t* __restrict__ tcell_off, uint32_t* __restrict__ flags, int mode,
+                  int32_t n, Vec4<T>* __restrict__ pos_o, Vec4<T>* __restrict__ vel_o,
+                  int32_t* __restrict__ id_o, int32_t* __restrict__ cell_o) {
   static_assert(kCellIds == 8, "the id block is read as two int4");
-  const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
-  if (p >= n) return;
-  const int32_t c = cell_of[p];
-  const int32_t me = id[p];
-  const Vec4<T> x = pos[p];
-  const Vec4<T> v = vel[p];
-  const uint32_t b = offset[c], e = offset[c + 1];
-  const uint32_t cnt = e - b;
-  const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
-  const int4 m0 = blk[0], m1 = blk[1];
-  const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-  uint32_t rank = 0;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    const bool active = p < p1;  // inactive lanes stay for the wave-wide mesh path
+    Vec4<T> x = {}, v = {};
+    int32_t c = 0;
+    uint32_t dst = 0;
+    int32_t me = 0;
+    int32_t k0 = 0, k1 = 0;  // triangle list of the particle's cell
+    const bool mesh = (mode & kModeContainer) && s.mesh >= 0;
+    if (active) {
+      c = cell_of[p];
+      me = id[p];
+      x = pos[p];
+      v = vel[p];
+      if (mesh) {
+        const int32_t tc = s.tcell_base + (c - s.grid.cell_base);
+        k0 = tcell_off[tc];
+        k1 = tcell_off[tc + 1];
+      }
+      const uint32_t b = offset[c], e = offset[c + 1];
+      const uint32_t cnt = e - b;
+      const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
+      const int4 m0 = blk[0], m1 = blk[1];
+      const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+      uint32_t rank = 0;
 #pragma unroll
-  for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
-  if (cnt > (uint32_t)kCellIds) {
-    const uint32_t len = ovf_ctr[1];
-    for (uint32_t o = 0; o < len; ++o) {
-      const int2 q = ovf[o];
-      rank += (q.x == c && q.y < me) ? 1u : 0u;
+      for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
+      if (cnt > (uint32_t)kCellIds) {
+        const uint32_t len = ovf_ctr[1];
+        for (uint32_t o = 0; o < len; ++o) {
+          const int2 q = ovf[o];
+          rank += (q.x == c && q.y < me) ? 1u : 0u;
+        }
+      }
+      // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
+      // arrays even if a stalled scan (reported through the flags) left them short
+      dst = min(b + rank, (uint32_t)(n - 1));
     }
-  }
-  // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
-  // arrays even if a stalled scan (reported through the flags) left them short
-  const uint32_t dst = min(b + rank, (uint32_t)(n - 1));
-  pos_o[dst] = x;
-  vel_o[dst] = v;
-  id_o[dst] = me;
-  if (cell_o) cell_o[dst] = c;
+    // The container acts on the pre-container state the bucketing hashed; its triangle-list loads
+    // run alongside the rank's offset and id-block loads.
+    if (mesh) {
+      mesh_contact_wave(&s, active, k0, k1, tent, x, v);
+    } else if (mode & kModeContainer) {
+      if (active) box_walls(&s, x, v);
+    }
+    if (!active) return;
+    if (mode & kModeCheck) {
+      const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
+                      isfinite(v.y) && isfinite(v.z);
+      if (!ok) atomicOr(flags, kFlagNonFinite);
+    }
+    pos_o[dst] = x;
+    vel_o[dst] = v;
+    id_o[dst] = me;
+    if (cell_o) cell_o[dst] = c;
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -600,27 +671,26 @@ __device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>*
   }
 }
 
-template <class T>
+template <class T, bool kTable>
 __global__ void __launch_bounds__(1024)
-    k_pairs(int32_t ntab, const BlockInfo* __restrict__ btab,
-            const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ pos,
+    k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
             Vec4<T>* __restrict__ vout) {
-  const int32_t tb = xcd_tile(blockIdx.x, ntab);
-  if (tb >= ntab) return;
-  const BlockInfo bi = btab[tb];
-  const int32_t p = bi.p_start + (int32_t)threadIdx.x;
-  if (p >= bi.p_end) return;
-  PairSelf<T> me;
-  uint32_t lo[9], hi[9];
-  pair_setup(scenes + bi.scene, pos, vsn, offset, p, me, lo, hi);
-  T ax = me.v.x, ay = me.v.y, az = me.v.z;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    if (p >= p1) return;
+    PairSelf<T> me;
+    uint32_t lo[9], hi[9];
+    pair_setup(&sc, pos, vsn, offset, p, me, lo, hi);
+    T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
-  for (int d = 0; d < 9; ++d) {
-    const uint32_t e = hi[d];
-    for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
-  }
-  vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+    for (int d = 0; d < 9; ++d) {
+      const uint32_t e = hi[d];
+      for (uint32_t q = lo[d]; q < e; q += 4)
+        pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
+    }
+    vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+  });
 }
 
 // The same contact sum with G lanes per particle (G divides 64, so a group never straddles a
@@ -631,44 +701,42 @@ __global__ void __launch_bounds__(1024)
 // bit pattern; lane 0 of the group writes it. More lanes per particle shorten the serial chain of
 // dependent loads per lane (about 81 candidates / G rounds), which is what bounds the kernel when
 // the particles alone are too few to fill the GPU.
-template <class T, int G>
+template <class T, int G, bool kTable>
 __global__ void __launch_bounds__(1024)
-    k_pairs_lanes(int32_t ntab, const BlockInfo* __restrict__ btab,
-                  const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ pos,
+    k_pairs_lanes(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
                   const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
                   Vec4<T>* __restrict__ vout) {
   static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "lanes per particle: 2, 4, 8 or 16");
-  const int32_t tb = xcd_tile(blockIdx.x, ntab);
-  if (tb >= ntab) return;
-  const BlockInfo bi = btab[tb];
-  const int32_t g = (int32_t)threadIdx.x % G;
-  const int32_t p = bi.p_start + (int32_t)threadIdx.x / G;
-  if (p >= bi.p_end) return;  // whole groups leave together
-  PairSelf<T> me;
-  uint32_t lo[9], hi[9];
-  pair_setup(scenes + bi.scene, pos, vsn, offset, p, me, lo, hi);
-  const int gbase = lane_id() & ~(G - 1);
-  T ax = me.v.x, ay = me.v.y, az = me.v.z;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
+    const int32_t g = (int32_t)threadIdx.x % G;
+    const int32_t p = p0 + (int32_t)threadIdx.x / G;
+    if (p >= p1) return;  // whole groups leave together
+    PairSelf<T> me;
+    uint32_t lo[9], hi[9];
+    pair_setup(&sc, pos, vsn, offset, p, me, lo, hi);
+    const int gbase = lane_id() & ~(G - 1);
+    T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
-  for (int d = 0; d < 9; ++d) {
-    const uint32_t e = hi[d];
-    for (uint32_t q0 = lo[d]; q0 < e; q0 += G) {  // the same trip count on every lane of a group
-      const uint32_t q = q0 + (uint32_t)g;
-      T fx = T(0), fy = T(0), fz = T(0);
-      bool hit = false;
-      if (q < e && (int32_t)q != p) hit = pair_kick(me, pos[q], vsn, q, fx, fy, fz);
-      uint32_t mine = (uint32_t)(__ballot(hit) >> gbase) & ((1u << G) - 1u);
-      while (mine) {  // identical on every lane of the group
-        const int src = gbase + __ffs(mine) - 1;
-        mine &= mine - 1u;
-        const T kx = __shfl(fx, src, 64), ky = __shfl(fy, src, 64), kz = __shfl(fz, src, 64);
-        ax = ax + kx;
-        ay = ay + ky;
-        az = az + kz;
+    for (int d = 0; d < 9; ++d) {
+      const uint32_t e = hi[d];
+      for (uint32_t q0 = lo[d]; q0 < e; q0 += G) {  // the same trip count on a whole group
+        const uint32_t q = q0 + (uint32_t)g;
+        T fx = T(0), fy = T(0), fz = T(0);
+        bool hit = false;
+        if (q < e && (int32_t)q != p) hit = pair_kick(me, pos[q], vsn, q, fx, fy, fz);
+        uint32_t mine = (uint32_t)(__ballot(hit) >> gbase) & ((1u << G) - 1u);
+        while (mine) {  // identical on every lane of the group
+          const int src = gbase + __ffs(mine) - 1;
+          mine &= mine - 1u;
+          const T kx = __shfl(fx, src, 64), ky = __shfl(fy, src, 64), kz = __shfl(fz, src, 64);
+          ax = ax + kx;
+          ay = ay + ky;
+          az = az + kz;
+        }
       }
     }
-  }
-  if (g == 0) vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+    if (g == 0) vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -725,15 +793,32 @@ __global__ void __launch_bounds__(256)
 static inline int32_t blocks_for(int64_t n, int b) { return (int32_t)((n + b - 1) / b); }
 
 template <class T>
+static Locator<T> make_locator(int ntab, int per_block, const BlockInfo* btab,
+                               const SceneDev<T>* scenes, const SceneDev<T>* one) {
+  Locator<T> loc;
+  loc.btab = btab;
+  loc.scenes = scenes;
+  loc.ntab = ntab;
+  loc.per_block = per_block;
+  if (one) loc.one = *one;
+  else std::memset(&loc.one, 0, sizeof(loc.one));
+  return loc;
+}
+
+template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                            const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of,
-                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
-                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
-                            const int32_t* tcell_off, uint32_t* flags, int mode) {
-  PSX_LAUNCH((k_integrate<T>), dim3(xcd_grid(ntab)), dim3(block), 0, st, ntab, btab, scenes,
-             pos, vel, id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, tgeo, tbox,
-             tcell_off, flags, mode);
+                            const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
+                            const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o,
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
+                            uint32_t* ovf_ctr, int mode) {
+  const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
+  if (one) {
+    PSX_LAUNCH((k_integrate<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
+               id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+  } else {
+    PSX_LAUNCH((k_integrate<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
+               id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+  }
   return hipGetLastError();
 }
 
@@ -746,42 +831,61 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 }
 
 template <class T>
-hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* offset, const int32_t* cell_ids, const int2* ovf,
-                              const uint32_t* ovf_ctr, const Vec4<T>* pos, const Vec4<T>* vel,
-                              const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+                              const SceneDev<T>* scenes, const SceneDev<T>* one, int32_t n,
+                              const int32_t* cell_of, const uint32_t* offset,
+                              const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
+                              const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
+                              const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
+                              int mode, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
                               int32_t* cell_o) {
-  const int32_t tiles = blocks_for(n, block);
-  PSX_LAUNCH((k_rank_gather<T>), dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of,
-             offset, cell_ids, ovf, ovf_ctr, pos, vel, id, pos_o, vel_o, id_o, cell_o);
+  const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
+  if (one) {
+    PSX_LAUNCH((k_rank_gather<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
+               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, n,
+               pos_o, vel_o, id_o, cell_o);
+  } else {
+    PSX_LAUNCH((k_rank_gather<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
+               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, n,
+               pos_o, vel_o, id_o, cell_o);
+  }
   return hipGetLastError();
 }
 
-template <class T>
-hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
-                        const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
-                        const uint32_t* offset, Vec4<T>* vout) {
-  const dim3 grid(xcd_grid(ntab)), blk(block);
+template <class T, bool kTable>
+static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int block, int lanes,
+                                 const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
+                                 Vec4<T>* vout) {
+  const dim3 grid(xcd_grid(loc.ntab)), blk(block);
   switch (lanes) {
     case 1:
-      PSX_LAUNCH((k_pairs<T>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
       break;
     case 2:
-      PSX_LAUNCH((k_pairs_lanes<T, 2>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
       break;
     case 4:
-      PSX_LAUNCH((k_pairs_lanes<T, 4>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 4, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
       break;
     case 8:
-      PSX_LAUNCH((k_pairs_lanes<T, 8>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 8, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
       break;
     case 16:
-      PSX_LAUNCH((k_pairs_lanes<T, 16>), grid, blk, 0, st, ntab, btab, scenes, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
+                        const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
+                        const Vec4<T>* vsn, const uint32_t* offset, Vec4<T>* vout) {
+  const Locator<T> loc = make_locator(ntab, block / lanes, btab, scenes, one);
+  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, vout);
+  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, vout);
 }
 
 template <class T>
@@ -802,19 +906,20 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
 
 #define PSX_INSTANTIATE(T)                                                                      \
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
-                                          const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,  \
-                                          const int32_t*, Vec4<T>*, Vec4<T>*, int32_t*,        \
-                                          uint32_t*, int32_t*, int2*, uint32_t*,               \
-                                          const TriGeo<T>*, const TriBox<T>*, const int32_t*,  \
-                                          uint32_t*, int);                                     \
-  template hipError_t launch_rank_gather<T>(hipStream_t, int, int32_t, const int32_t*,          \
-                                            const uint32_t*, const int32_t*, const int2*,      \
-                                            const uint32_t*, const Vec4<T>*, const Vec4<T>*,   \
-                                            const int32_t*, Vec4<T>*, Vec4<T>*, int32_t*,      \
-                                            int32_t*);                                         \
+                                          const SceneDev<T>*, const SceneDev<T>*,              \
+                                          const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
+                                          Vec4<T>*, Vec4<T>*, int32_t*, uint32_t*, int32_t*,   \
+                                          int2*, uint32_t*, int);                              \
+  template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
+                                            const SceneDev<T>*, const SceneDev<T>*, int32_t,   \
+                                            const int32_t*, const uint32_t*, const int32_t*,   \
+                                            const int2*, const uint32_t*, const Vec4<T>*,      \
+                                            const Vec4<T>*, const int32_t*,                    \
+                                            const TriEntry<T>*, const int32_t*, uint32_t*,     \
+                                            int, Vec4<T>*, Vec4<T>*, int32_t*, int32_t*);      \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
-                                      const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
-                                      const uint32_t*, Vec4<T>*);                              \
+                                      const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
+                                      const Vec4<T>*, const uint32_t*, Vec4<T>*);              \
   template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,    \
                                        const int32_t*, T*, T*);                                \
   template hipError_t launch_sort_in<T>(hipStream_t, int32_t, Vec4<T>*, Vec4<T>*,               \

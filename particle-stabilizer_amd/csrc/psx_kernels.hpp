@@ -15,19 +15,31 @@ constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the overflow list
+// Largest workgroup of the rank/gather kernel: its binary64 form (rank, walls and the mesh contact
+// in double) needs more than the 128 registers a 1024-wide workgroup leaves per lane.
+template <class T>
+constexpr int kMaxBlockSort = sizeof(T) == 4 ? 1024 : 512;
+
+// Spare entries after the triangle-list entry array (the container kernel reads list entries four
+// at a time).
+constexpr int kTriPad = 4;
+
 // Spare entries after every sorted position array the pair kernel reads (its chunked loads may
 // reach three entries past the last particle).
 constexpr int kPairPad = 4;
 
-// cell_ids: kCellIds slots per cell (C_pad * kCellIds); ovf: n entries (cell, id); ovf_ctr[0]
-// counts the overflow entries of this pass and must be 0 on entry (k_scan resets it).
+// Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
+// (ntab = workgroups); otherwise btab holds one (scene, slice) entry per workgroup.
+//
+// Integrate (kModeIntegrate: writes the integrated, pre-container state to pos_o/vel_o), hash,
+// count. cell_ids: kCellIds slots per cell (C_pad * kCellIds); ovf: n entries (cell, id);
+// ovf_ctr[0] counts the overflow entries of this pass and must be 0 on entry (k_scan resets it).
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                            const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* cell_of,
-                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
-                            const TriGeo<T>* tgeo, const TriBox<T>* tbox,
-                            const int32_t* tcell_off, uint32_t* flags, int mode);
+                            const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
+                            const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o,
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
+                            uint32_t* ovf_ctr, int mode);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
 // (the last entry is the total); clears the counts, moves ovf_ctr[0] to ovf_ctr[1] and zeroes
@@ -37,18 +49,23 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
                        unsigned long long* ticket, int32_t ntiles, uint32_t* ovf_ctr,
                        uint32_t* flags);
 
+// Stable rank among cell mates, container (kModeContainer: walls or mesh) and non-finite check
+// (kModeCheck) on the state (pos, vel), gather into cell-sorted order.
 template <class T>
-hipError_t launch_rank_gather(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* offset, const int32_t* cell_ids, const int2* ovf,
-                              const uint32_t* ovf_ctr, const Vec4<T>* pos, const Vec4<T>* vel,
-                              const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+                              const SceneDev<T>* scenes, const SceneDev<T>* one, int32_t n,
+                              const int32_t* cell_of, const uint32_t* offset,
+                              const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
+                              const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
+                              const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
+                              int mode, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
                               int32_t* cell_o);
 
-// lanes: lanes per particle (1, 2, 4, 8 or 16); btab holds block / lanes particles per workgroup.
+// lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
 template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
-                        const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vsn,
-                        const uint32_t* offset, Vec4<T>* vout);
+                        const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
+                        const Vec4<T>* vsn, const uint32_t* offset, Vec4<T>* vout);
 
 template <class T>
 hipError_t launch_unsort(hipStream_t st, int32_t n, const Vec4<T>* pos, const Vec4<T>* vel,

@@ -237,7 +237,8 @@ struct SimImpl final : SimBase {
   DevBuf pos[2], vel[2], id[2];
   DevBuf cell_of, cell_ids, ovf, ovf_ctr;
   DevBuf count, offset, tile_state, ticket, flags;
-  DevBuf scenes_d, tgeo_d, tbox_d, tcell_off_d;
+  DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d;
+  std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
   std::map<int, std::unique_ptr<DevBuf>> btab;
   std::map<int, int> btab_blocks;
@@ -297,6 +298,7 @@ struct SimImpl final : SimBase {
     std::vector<SceneDev<T>> sd(scenes.size());
     std::vector<TriGeo<T>> tgeo;
     std::vector<TriBox<T>> tbox;
+    std::vector<TriEntry<T>> tent;
     std::vector<int32_t> toff;
     for (size_t s = 0; s < scenes.size(); ++s) {
       SceneHost& sh = scenes[s];
@@ -377,7 +379,18 @@ struct SimImpl final : SimBase {
         }
         const int32_t entry_base = (int32_t)tbox.size();
         for (size_t c2 = 0; c2 + 1 < m.off.size(); ++c2) toff.push_back(entry_base + m.off[c2]);
-        for (int32_t t : m.list) tbox.push_back(boxes[(size_t)t]);
+        for (int32_t t : m.list) {
+          const TriBox<T>& bx = boxes[(size_t)t];
+          tbox.push_back(bx);
+          TriEntry<T> te;
+          std::memset(&te, 0, sizeof(te));
+          te.g = tgeo[(size_t)bx.tri];
+          for (int a = 0; a < 3; ++a) {
+            te.lo[a] = bx.lo[a];
+            te.hi[a] = bx.hi[a];
+          }
+          tent.push_back(te);
+        }
         toff.push_back((int32_t)tbox.size());  // end of this scene's last cell
       }
     }
@@ -404,6 +417,7 @@ struct SimImpl final : SimBase {
     scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
     tgeo_d.alloc(std::max<size_t>(tgeo.size(), 1) * sizeof(TriGeo<T>));
     tbox_d.alloc(std::max<size_t>(tbox.size(), 1) * sizeof(TriBox<T>));
+    tent_d.alloc((tent.size() + kTriPad) * sizeof(TriEntry<T>));
     tcell_off_d.alloc(std::max<size_t>(toff.size(), 1) * 4);
 
     PSX_HIP(hipMemcpy(pos[0].p, p4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
@@ -417,16 +431,21 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemset(ticket.p, 0, ticket.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
     PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
+    scenes_h = sd;
     if (!tgeo.empty())
       PSX_HIP(hipMemcpy(tgeo_d.p, tgeo.data(), tgeo.size() * sizeof(TriGeo<T>), hipMemcpyHostToDevice));
     if (!tbox.empty())
       PSX_HIP(hipMemcpy(tbox_d.p, tbox.data(), tbox.size() * sizeof(TriBox<T>), hipMemcpyHostToDevice));
+    PSX_HIP(hipMemset(tent_d.p, 0, tent_d.bytes));  // the padding entries are read, never used
+    if (!tent.empty())
+      PSX_HIP(hipMemcpy(tent_d.p, tent.data(), tent.size() * sizeof(TriEntry<T>), hipMemcpyHostToDevice));
     if (!toff.empty())
       PSX_HIP(hipMemcpy(tcell_off_d.p, toff.data(), toff.size() * 4, hipMemcpyHostToDevice));
     tri_count = (int64_t)tgeo.size();
     tcell_entries = (int64_t)tbox.size();
     int nb;
     table(tuning.block_integrate, &nb);
+    table(tuning.block_sort, &nb);
     table(pair_particles_per_block(), &nb);
     PSX_HIP(hipDeviceSynchronize());
   }
@@ -441,24 +460,26 @@ struct SimImpl final : SimBase {
     PSX_HIP(e);
   }
 
-  // Bucketing: k_integrate in `mode` reads (ps, vs) and, when it integrates or applies the
-  // container, writes the result to (po, vo) (which may alias ps, vs) and files the ids `is` into
-  // the cell id blocks; scan into `off`; stable rank and gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
+  // Bucketing: k_integrate reads (ps, vs), integrates when `mode` says so (writing the
+  // pre-container state to (po, vo), which may alias (ps, vs)), hashes and files the ids `is` into
+  // the cell id blocks; scan into `off`; stable rank, container (kModeContainer) and check
+  // (kModeCheck), gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
   // cell_sorted when it is not NULL.
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, Vec4<T>* po, Vec4<T>* vo, const int32_t* is,
               Vec4<T>* pd, Vec4<T>* vd, int32_t* idd, uint32_t* off, int32_t* cell_sorted,
               int mode, Profiler* prof) {
-    int nb_i;
+    int nb_i, nb_s;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
+    const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
+    const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
     const int32_t nn = (int32_t)n;
-    const bool writes = (mode & (kModeIntegrate | kModeContainer)) != 0;
+    const bool integrates = (mode & kModeIntegrate) != 0;
     launch(0, prof, [&] {
-      return launch_integrate<T>(stream, nb_i, tuning.block_integrate, bt_i,
-                                 scenes_d.as<SceneDev<T>>(), ps, vs, is, po, vo,
+      return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
+                                 scenes_d.as<SceneDev<T>>(), one, ps, vs, is, po, vo,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                 tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
-                                 tcell_off_d.as<int32_t>(), flags.as<uint32_t>(), mode);
+                                 mode);
     });
     launch(1, prof, [&] {
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(),
@@ -466,10 +487,12 @@ struct SimImpl final : SimBase {
                          flags.as<uint32_t>());
     });
     launch(2, prof, [&] {
-      return launch_rank_gather<T>(stream, tuning.block_sort, nn, cell_of.as<int32_t>(), off,
+      return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
+                                   scenes_d.as<SceneDev<T>>(), one, nn, cell_of.as<int32_t>(), off,
                                    cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                   writes ? po : ps, writes ? vo : vs, is, pd, vd, idd,
-                                   cell_sorted);
+                                   integrates ? po : ps, integrates ? vo : vs, is,
+                                   tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
+                                   flags.as<uint32_t>(), mode, pd, vd, idd, cell_sorted);
     });
   }
 
@@ -481,6 +504,9 @@ struct SimImpl final : SimBase {
            id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr, mode, prof);
   }
 
+  // A single scene launches without the workgroup table (constants by value, slices by index).
+  bool single() const { return scenes_h.size() == 1; }
+
   // Lanes per particle of the pair kernel (tuning.pair_variant; 0 means 1).
   int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
   int pair_particles_per_block() const { return tuning.block_pairs / pair_lanes(); }
@@ -490,8 +516,9 @@ struct SimImpl final : SimBase {
     int nb_p;
     const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
     launch(3, prof, [&] {
-      return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(), bt_p,
-                             scenes_d.as<SceneDev<T>>(), ps, vs, off, vout);
+      return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
+                             single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
+                             single() ? scenes_h.data() : nullptr, ps, vs, off, vout);
     });
   }
 
@@ -713,11 +740,11 @@ struct SimImpl final : SimBase {
         throw Error(PSX_ERR_INVALID, "psx_set_params cannot change the container bounds");
     if (p.dtype != sh.prm.dtype) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change dtype");
     sh.prm = p;
-    SceneDev<T> d;
     PSX_HIP(hipStreamSynchronize(stream));
-    PSX_HIP(hipMemcpy(&d, scenes_d.as<SceneDev<T>>() + scene, sizeof(d), hipMemcpyDeviceToHost));
+    SceneDev<T>& d = scenes_h[(size_t)scene];
     fill_coefficients(d, sh.prm);
     PSX_HIP(hipMemcpy(scenes_d.as<SceneDev<T>>() + scene, &d, sizeof(d), hipMemcpyHostToDevice));
+    destroy_graphs();  // single-scene launches carry the constants by value in captured graphs
   }
 
   static void fill_coefficients(SceneDev<T>& d, const psx_params& prm) {
@@ -796,6 +823,7 @@ struct SimImpl final : SimBase {
     tuning = t;
     int nb;
     table(tuning.block_integrate, &nb);
+    table(tuning.block_sort, &nb);
     table(pair_particles_per_block(), &nb);
   }
 
@@ -1088,6 +1116,8 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
     if (b < 64 || b > 1024 || (b % 64) != 0)
       return set_error(PSX_ERR_INVALID, "block sizes must be multiples of 64 in [64, 1024]");
   if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
+  if (sim->impl->dtype() == PSX_F64 && c.block_sort > psx::kMaxBlockSort<double>)
+    return set_error(PSX_ERR_INVALID, "block_sort must be at most 512 for a binary64 simulation");
   if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
       c.pair_variant != 8 && c.pair_variant != 16)
     return set_error(PSX_ERR_INVALID, "pair_variant (lanes per particle) must be 0, 1, 2, 4, 8 or 16");

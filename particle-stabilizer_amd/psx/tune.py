@@ -6,7 +6,7 @@ through the shorter lists), profiles a few steps with the kernels' own dispatch 
 each kernel keeps the shape with the smallest median time over its rounds. Tuned knobs:
 
 - block_integrate: workgroup size of the integrate / hash / container kernel;
-- block_sort: workgroup size of the rank / gather kernel;
+- block_sort: workgroup size of the rank / gather / container kernel (at most 512 in binary64);
 - (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel.
 
 The cell size stays what the reference's grid rule fixes (it changes the neighbour search, not
@@ -60,9 +60,11 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
     best: dict[str, tuple[float, object]] = {}
     n_rounds = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
     samples: dict[tuple[str, object], list] = {}
+    # the binary64 rank/gather kernel takes workgroups of at most 512
+    sort_blocks = BLOCKS if np.dtype(sim.dtype) == np.float32 else tuple(b for b in BLOCKS if b <= 512)
     for k in range(n_rounds):
         bi = BLOCKS[k % len(BLOCKS)]
-        bs = BLOCKS[(k // len(BLOCKS)) % len(BLOCKS)]
+        bs = sort_blocks[(k // len(BLOCKS)) % len(sort_blocks)]
         bp, g = PAIR_SHAPES[k % len(PAIR_SHAPES)]
         sim.set_tuning(block_integrate=bi, block_sort=bs, block_pairs=bp, pair_variant=g,
                        use_graph=0)
