@@ -32,7 +32,29 @@ struct GridDev {
   double h;
   int32_t n[3];
   int32_t cell_base;
+  // division by n[2] and n[1] as multiply-high and shifts (Granlund and Montgomery, "Division by
+  // invariant integers using multiplication", 1994, fig. 4.1): magic[a] and shift[a] = s1 | s2 << 8
+  // for a = 0 (n[2]) and 1 (n[1])
+  uint32_t magic[2];
+  uint32_t shift[2];
 };
+
+// floor(u / d) for the divisor behind (m, sh): t = mulhi(m, u); (t + ((u - t) >> s1)) >> s2.
+__device__ __forceinline__ uint32_t div_invariant(uint32_t u, uint32_t m, uint32_t sh) {
+  const uint32_t t = __umulhi(m, u);
+  return (t + ((u - t) >> (sh & 0xffu))) >> (sh >> 8);
+}
+
+// (ix, iy, iz) of cell c of grid g.
+__device__ __forceinline__ void cell_coords(const GridDev& g, int32_t c, int32_t& ix, int32_t& iy,
+                                            int32_t& iz) {
+  const uint32_t u = (uint32_t)(c - g.cell_base);
+  const uint32_t t = div_invariant(u, g.magic[0], g.shift[0]);  // u / nz
+  iz = (int32_t)(u - t * (uint32_t)g.n[2]);
+  const uint32_t x = div_invariant(t, g.magic[1], g.shift[1]);  // t / ny
+  iy = (int32_t)(t - x * (uint32_t)g.n[1]);
+  ix = (int32_t)x;
+}
 
 // Per-scene constants on the device, in the state's arithmetic type T.
 template <class T>

@@ -2,14 +2,15 @@
 //
 // One step (PhysicsEngine.cpp:38-48, CPU order: integrate, bucket from the pre-container positions,
 // container, pairs) is these launches on one stream:
-//   k_integrate    x += v*dt; v += g*dt; cell hash of the new (pre-container) x; per-cell count
+//   k_integrate    x += v*dt; cell hash of the new (pre-container) x; per-cell count
 //                  (atomic, returns the particle's arrival slot), and the particle's id written to
 //                  that slot of its cell's id block (kCellIds slots; arrivals past them go to an
 //                  overflow list)
 //   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
 //                  look-back over tiles, wave64 scans inside a tile); clears the counts
-//   k_rank_gather  rank of each particle among its cell mates by id (stable order); box walls or
-//                  mesh contact on the particle; gather of the result into cell-sorted order
+//   k_rank_gather  rank of each particle among its cell mates by id (stable order); x += v*dt;
+//                  v += g*dt (the same operations again); box walls or mesh contact; gather of the
+//                  result into cell-sorted order
 //   k_pairs        27-cell spring/restitution velocity kick, Jacobi over the sorted snapshot
 // Particle state lives in cell-sorted order between steps; an id array carries original indices.
 //
@@ -282,31 +283,43 @@ __device__ __forceinline__ void mesh_contact_wave(const SceneDev<T>* __restrict_
   }
 }
 
+// PhysicsEngine::updatePositionAndVelocity (PhysicsEngine.cpp:22-25): x += v dt with the old v,
+// then v += g dt. The hash kernel needs the new positions, the rank/gather kernel the whole state;
+// both evaluate the same operations, so they agree bit for bit.
+template <class T>
+__device__ __forceinline__ void integrate_position(const SceneDev<T>& s, Vec4<T>& x,
+                                                   const Vec4<T>& v) {
+  const T dt = s.dt;
+  x.x = x.x + v.x * dt;
+  x.y = x.y + v.y * dt;
+  x.z = x.z + v.z * dt;
+}
+
+template <class T>
+__device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>& v) {
+  v.x = v.x + s.gdt[0];
+  v.y = v.y + s.gdt[1];
+  v.z = v.z + s.gdt[2];
+}
+
 // ---------------------------------------------------------------------------------------------
 // K1: integrate + cell hash + container + counts
 // ---------------------------------------------------------------------------------------------
 template <class T, bool kTable>
 __global__ void __launch_bounds__(1024)
-    k_integrate(const Locator<T> loc, const Vec4<T>* pos, const Vec4<T>* vel,
-                const int32_t* __restrict__ id, Vec4<T>* pos_o, Vec4<T>* vel_o,
+    k_integrate(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
+                const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
                 uint32_t* __restrict__ ovf_ctr, int mode) {
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     if (p >= p1) return;
-    Vec4<T> x = pos[p], v = vel[p];
+    Vec4<T> x = pos[p];
     const int32_t me = id[p];
-    if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25
-      const T dt = s.dt;
-      x.x = x.x + v.x * dt;
-      x.y = x.y + v.y * dt;
-      x.z = x.z + v.z * dt;
-      v.x = v.x + s.gdt[0];
-      v.y = v.y + s.gdt[1];
-      v.z = v.z + s.gdt[2];
-      pos_o[p] = x;
-      vel_o[p] = v;
+    if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25 (positions only: the hash needs them)
+      const Vec4<T> v = vel[p];
+      integrate_position(s, x, v);
     }
     // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container x.
     const GridDev& g = s.grid;
@@ -333,13 +346,17 @@ __global__ void __launch_bounds__(1024)
 // published inclusive prefix, publishes its own inclusive prefix, writes the offsets and clears
 // the counts for the next step.
 //
-// Forward progress: tiles are handed out in the order workgroups start (a ticket counter), so a
-// workgroup only waits for tiles held by workgroups that started before it and never wait for a
-// later one; tile 0 waits for nobody. Status words carry the launch's epoch (ticket / ntiles + 1),
-// so the words left by the previous launch are never mistaken for current ones and nothing has to
-// be cleared between launches. Every wait is bounded: past kScanSpinLimit polls the predecessor is
-// taken as a prefix of 0, the stall flag is raised for the host, and all offsets stay at or below
-// their true values (so every index derived from them stays inside its array).
+// Forward progress: workgroup b scans tile b and waits only for tiles below b. Workgroups are
+// dealt to the XCDs round-robin and every XCD dispatches its share in increasing index order, so
+// the lowest unfinished tile has always been dispatched (every workgroup resident on its XCD has a
+// lower index and has finished, or it would not be the lowest) and waits for finished tiles only;
+// tile 0 waits for nobody. Status words carry an epoch: each tile's word holds the epoch of the
+// last launch that scanned it, and a launch uses that epoch + 1 (every launch scans every tile, so
+// all words agree between launches). The words left by the previous launch are never mistaken for
+// current ones and nothing has to be cleared between launches. Every wait is bounded: past
+// kScanSpinLimit polls the predecessor is taken as a prefix of 0, the stall flag is raised for the
+// host, and all offsets stay at or below their true values (so every index derived from them stays
+// inside its array).
 // ---------------------------------------------------------------------------------------------
 constexpr uint64_t kTileAggregate = 1, kTilePrefix = 2;
 constexpr uint32_t kEpochMask = (1u << 30) - 1;
@@ -351,18 +368,16 @@ __device__ inline uint64_t tile_word(uint32_t value, uint64_t status, uint32_t e
 
 __global__ void __launch_bounds__(kScanThreads)
     k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
-           uint64_t* __restrict__ tile_state, unsigned long long* __restrict__ ticket,
-           int32_t ntiles, uint32_t* __restrict__ ovf_ctr, uint32_t* __restrict__ flags) {
+           uint64_t* __restrict__ tile_state, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
+           uint32_t* __restrict__ flags) {
   __shared__ uint32_t wsum[kScanThreads / 64];
-  __shared__ uint32_t s_tile, s_epoch, s_prefix;
-  if (threadIdx.x == 0) {
-    const unsigned long long t = atomicAdd(ticket, 1ull);
-    s_tile = (uint32_t)(t % (unsigned long long)ntiles);
-    s_epoch = (uint32_t)((t / (unsigned long long)ntiles + 1ull) & kEpochMask);
-  }
-  __syncthreads();
-  const int32_t tile = (int32_t)s_tile;
-  const uint32_t epoch = s_epoch;
+  __shared__ uint32_t s_prefix;
+  const int32_t tile = (int32_t)blockIdx.x;
+  // this tile's word from the previous launch: its epoch + 1 is this launch's (read alongside the
+  // counts; only this workgroup writes the word)
+  const uint32_t epoch =
+      ((uint32_t)(__hip_atomic_load(tile_state + tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >>
+                  34) + 1u) & kEpochMask;
   if (tile == 0 && threadIdx.x == 0) {
     // the overflow list of the cell id blocks is complete (written by the previous kernel):
     // publish its length for the rank/gather kernel and restart the counter for the next pass
@@ -468,8 +483,9 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
                   const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
                   const int32_t* __restrict__ id, const TriEntry<T>* __restrict__ tent,
                   const int32_t* __restrict__ tcell_off, uint32_t* __restrict__ flags, int mode,
-                  int32_t n, Vec4<T>* __restrict__ pos_o, Vec4<T>* __restrict__ vel_o,
-                  int32_t* __restrict__ id_o, int32_t* __restrict__ cell_o) {
+                  const uint8_t* __restrict__ cull, int32_t n, Vec4<T>* __restrict__ pos_o,
+                  Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
+                  int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o) {
   static_assert(kCellIds == 8, "the id block is read as two int4");
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -479,17 +495,23 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     uint32_t dst = 0;
     int32_t me = 0;
     int32_t k0 = 0, k1 = 0;  // triangle list of the particle's cell
+    uint8_t cull_c = 0;      // cull flag of the bucket cell
     const bool mesh = (mode & kModeContainer) && s.mesh >= 0;
     if (active) {
       c = cell_of[p];
       me = id[p];
       x = pos[p];
       v = vel[p];
+      if (mode & kModeIntegrate) {
+        integrate_position(s, x, v);
+        integrate_velocity(s, v);
+      }
       if (mesh) {
         const int32_t tc = s.tcell_base + (c - s.grid.cell_base);
         k0 = tcell_off[tc];
         k1 = tcell_off[tc + 1];
       }
+      cull_c = cull[c];
       const uint32_t b = offset[c], e = offset[c + 1];
       const uint32_t cnt = e - b;
       const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
@@ -511,12 +533,25 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     }
     // The container acts on the pre-container state the bucketing hashed; its triangle-list loads
     // run alongside the rank's offset and id-block loads.
+    const T x0 = x.x, y0 = x.y, z0 = x.z;
     if (mesh) {
       mesh_contact_wave(&s, active, k0, k1, tent, x, v);
     } else if (mode & kModeContainer) {
       if (active) box_walls(&s, x, v);
     }
     if (!active) return;
+    // home cell of the post-container position for the pair kernel (the bucket cell unless the
+    // container moved the particle), with the cell's cull flag in bit 31
+    int32_t home = c;
+    if (!(x.x == x0 && x.y == y0 && x.z == z0)) {
+      const GridDev& g = s.grid;
+      const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+      const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+      const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+      home = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+      cull_c = cull[home];
+    }
+    home_o[dst] = (uint32_t)home | (cull_c ? kHomeCull : 0u);
     if (mode & kModeCheck) {
       const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
                       isfinite(v.y) && isfinite(v.z);
@@ -601,23 +636,62 @@ __device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4
 }
 
 // Particle i's constants and the nine column slices [lo[d], hi[d]) of its 27 neighbour cells.
+//
+// Cell culling. A cell is bounded when every member's snapshot position lies inside the cell's box:
+// interior cells (not on the grid boundary, where positions outside the domain are clamped in)
+// whose triangle list is empty (no container push can move a member: a pushed particle touched a
+// triangle within reach of its bucket cell, and box walls only push particles of boundary cells).
+// cull[c] marks cells whose 27 neighbours are all bounded; for such a home cell, neighbour cells
+// whose box is farther from x_i than reach = r_i + r_max (enlarged by a margin for rounding) hold
+// no contact of i and are dropped from the column slices. Only non-contacts are skipped, so the
+// contacts and their order are those of the full walk.
 template <class T>
 __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
                                            const Vec4<T>* __restrict__ pos,
                                            const Vec4<T>* __restrict__ vsn,
-                                           const uint32_t* __restrict__ offset, int32_t p,
+                                           const uint32_t* __restrict__ offset,
+                                           const uint32_t* __restrict__ home_of, int32_t p,
                                            PairSelf<T>& me, uint32_t (&lo)[9], uint32_t (&hi)[9]) {
   const GridDev& g = s->grid;
   me.x = pos[p];
   me.v = vsn[p];
+  // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped) and its cull flag,
+  // from the gather
+  const uint32_t hw = home_of[p];
+  const int32_t home = (int32_t)(hw & ~kHomeCull);
   const T rr = me.x.w + s->rmax;
   me.thr = (rr * rr) * PairMargin<T>::value;
   me.ks = s->spring;
   me.es = s->es;
-  // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped)
-  const int32_t hx = axis_index((double)me.x.x, g.min[0], g.h, g.n[0]);
-  const int32_t hy = axis_index((double)me.x.y, g.min[1], g.h, g.n[1]);
-  const int32_t hz = axis_index((double)me.x.z, g.min[2], g.h, g.n[2]);
+  int32_t hx, hy, hz;
+  cell_coords(g, home, hx, hy, hz);
+  if (hw & kHomeCull) {  // all 27 neighbours exist and are bounded
+    // distances (binary64) from x_i to the lower and upper neighbour along each axis
+    const double ux = ((double)me.x.x - g.min[0]) - (double)hx * g.h;
+    const double uy = ((double)me.x.y - g.min[1]) - (double)hy * g.h;
+    const double uz = ((double)me.x.z - g.min[2]) - (double)hz * g.h;
+    const double dlo[3] = {fmax(ux, 0.0), fmax(uy, 0.0), fmax(uz, 0.0)};
+    const double dhi[3] = {fmax(g.h - ux, 0.0), fmax(g.h - uy, 0.0), fmax(g.h - uz, 0.0)};
+    const double reach = ((double)me.x.w + (double)s->rmax) * (1.0 + 1e-4) + 1e-4 * g.h;
+    const double r2 = reach * reach;
+    const double zm = dlo[2] * dlo[2], zp = dhi[2] * dhi[2];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const int dx = d / 3 - 1, dy = d % 3 - 1;
+      const double ex = dx < 0 ? dlo[0] : (dx > 0 ? dhi[0] : 0.0);
+      const double ey = dy < 0 ? dlo[1] : (dy > 0 ? dhi[1] : 0.0);
+      const double exy = ex * ex + ey * ey;
+      lo[d] = 0;
+      hi[d] = 0;
+      if (exy <= r2) {  // the column's middle cell is within reach; its ends may be
+        const uint32_t* __restrict__ col =
+            offset + home + (dx * g.n[1] + dy) * g.n[2];  // cell (hx+dx, hy+dy, hz)
+        lo[d] = col[exy + zm <= r2 ? -1 : 0];
+        hi[d] = col[exy + zp <= r2 ? 2 : 1];
+      }
+    }
+    return;
+  }
   const int32_t zlo = max(hz - 1, 0), zhi = min(hz + 1, g.n[2] - 1);
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
@@ -634,15 +708,17 @@ __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
 
 // Candidates are taken four at a time: one address, four independent loads, four prefilters and a
 // single branch into the rare path, which applies the contacts of the chunk in ascending order.
-// The loads may run up to three entries past the end of a slice; the sorted position arrays carry
-// kPairPad spare entries so that they stay inside the allocation, and those entries are masked out.
+// The loads may run up to three entries past the end of a slice (or read four entries of an empty
+// slice at offset 0); the sorted position arrays carry kPairPad spare entries so that they stay
+// inside the allocation, and those entries are masked out.
+// Tests the chunk c0..c3 = candidates q .. q+3 of a slice ending at e.
 template <class T>
-__device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
-                                           const Vec4<T>* __restrict__ vsn, uint32_t q, uint32_t e,
-                                           uint32_t self, T& ax, T& ay, T& az) {
-  const Vec4<T>* __restrict__ src = pos + q;  // one address; the other three are offsets
-  const Vec4<T> c0 = src[0], c1 = src[1], c2 = src[2], c3 = src[3];
-  const bool m0 = q != self && pair_near(me, c0);
+__device__ __forceinline__ void pair_test4(const PairSelf<T>& me, const Vec4<T>& c0,
+                                           const Vec4<T>& c1, const Vec4<T>& c2,
+                                           const Vec4<T>& c3, const Vec4<T>* __restrict__ vsn,
+                                           uint32_t q, uint32_t e, uint32_t self, T& ax, T& ay,
+                                           T& az) {
+  const bool m0 = q < e && q != self && pair_near(me, c0);
   const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
   const bool m2 = q + 2 < e && q + 2 != self && pair_near(me, c2);
   const bool m3 = q + 3 < e && q + 3 != self && pair_near(me, c3);
@@ -671,24 +747,61 @@ __device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>*
   }
 }
 
+template <class T>
+__device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                           const Vec4<T>* __restrict__ vsn, uint32_t q, uint32_t e,
+                                           uint32_t self, T& ax, T& ay, T& az) {
+  const Vec4<T>* __restrict__ src = pos + q;  // one address; the other three are offsets
+  pair_test4(me, src[0], src[1], src[2], src[3], vsn, q, e, self, ax, ay, az);
+}
+
+// The contact sum of particle p over its nine column slices, in slice order. In binary32 the first
+// chunk of the next slice is loaded while the current slice is tested, so the slices' loads
+// overlap instead of forming a chain of nine round trips (binary64 lacks the registers for it).
+template <class T>
+__device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                         const Vec4<T>* __restrict__ vsn,
+                                         const uint32_t (&lo)[9], const uint32_t (&hi)[9],
+                                         uint32_t self, T& ax, T& ay, T& az) {
+  if constexpr (sizeof(T) == 4) {
+    Vec4<T> n0 = pos[lo[0]], n1 = pos[lo[0] + 1], n2 = pos[lo[0] + 2], n3 = pos[lo[0] + 3];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const Vec4<T> c0 = n0, c1 = n1, c2 = n2, c3 = n3;
+      if (d + 1 < 9) {
+        const Vec4<T>* __restrict__ nx = pos + lo[d + 1];
+        n0 = nx[0];
+        n1 = nx[1];
+        n2 = nx[2];
+        n3 = nx[3];
+      }
+      const uint32_t e = hi[d];
+      // unconditional (the masks cover an empty slice), so the prefetched chunk is used as loaded
+      pair_test4(me, c0, c1, c2, c3, vsn, lo[d], e, self, ax, ay, az);
+      for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const uint32_t e = hi[d];
+      for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+    }
+  }
+}
+
 template <class T, bool kTable>
 __global__ void __launch_bounds__(1024)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
-            Vec4<T>* __restrict__ vout) {
+            const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout) {
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     if (p >= p1) return;
     PairSelf<T> me;
     uint32_t lo[9], hi[9];
-    pair_setup(&sc, pos, vsn, offset, p, me, lo, hi);
+    pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
     T ax = me.v.x, ay = me.v.y, az = me.v.z;
-#pragma unroll
-    for (int d = 0; d < 9; ++d) {
-      const uint32_t e = hi[d];
-      for (uint32_t q = lo[d]; q < e; q += 4)
-        pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
-    }
+    pair_sum(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
     vout[p] = Vec4<T>{ax, ay, az, me.v.w};
   });
 }
@@ -705,7 +818,7 @@ template <class T, int G, bool kTable>
 __global__ void __launch_bounds__(1024)
     k_pairs_lanes(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
                   const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
-                  Vec4<T>* __restrict__ vout) {
+                  const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout) {
   static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "lanes per particle: 2, 4, 8 or 16");
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t g = (int32_t)threadIdx.x % G;
@@ -713,7 +826,7 @@ __global__ void __launch_bounds__(1024)
     if (p >= p1) return;  // whole groups leave together
     PairSelf<T> me;
     uint32_t lo[9], hi[9];
-    pair_setup(&sc, pos, vsn, offset, p, me, lo, hi);
+    pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
     const int gbase = lane_id() & ~(G - 1);
     T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
@@ -808,25 +921,24 @@ static Locator<T> make_locator(int ntab, int per_block, const BlockInfo* btab,
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                            const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o,
-                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, int mode) {
+                            const Vec4<T>* vel, const int32_t* id, int32_t* cell_of,
+                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
+                            int mode) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   if (one) {
     PSX_LAUNCH((k_integrate<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
-               id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+               id, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
   } else {
     PSX_LAUNCH((k_integrate<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
-               id, pos_o, vel_o, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+               id, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       unsigned long long* ticket, int32_t ntiles, uint32_t* ovf_ctr,
-                       uint32_t* flags) {
-  PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ticket,
-             ntiles, ovf_ctr, flags);
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags) {
+  PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ntiles,
+             ovf_ctr, flags);
   return hipGetLastError();
 }
 
@@ -837,17 +949,17 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
                               const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                               const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
-                              int mode, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o) {
+                              int mode, const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o,
+                              int32_t* id_o, int32_t* cell_o, uint32_t* home_o) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   if (one) {
     PSX_LAUNCH((k_rank_gather<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
-               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, n,
-               pos_o, vel_o, id_o, cell_o);
+               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, cull,
+               n, pos_o, vel_o, id_o, cell_o, home_o);
   } else {
     PSX_LAUNCH((k_rank_gather<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
-               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, n,
-               pos_o, vel_o, id_o, cell_o);
+               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, cull,
+               n, pos_o, vel_o, id_o, cell_o, home_o);
   }
   return hipGetLastError();
 }
@@ -855,23 +967,23 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
 template <class T, bool kTable>
 static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int block, int lanes,
                                  const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
-                                 Vec4<T>* vout) {
+                                 const uint32_t* home, Vec4<T>* vout) {
   const dim3 grid(xcd_grid(loc.ntab)), blk(block);
   switch (lanes) {
     case 1:
-      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
       break;
     case 2:
-      PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
       break;
     case 4:
-      PSX_LAUNCH((k_pairs_lanes<T, 4, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 4, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
       break;
     case 8:
-      PSX_LAUNCH((k_pairs_lanes<T, 8, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 8, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
       break;
     case 16:
-      PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
       break;
     default:
       return hipErrorInvalidValue;
@@ -882,10 +994,11 @@ static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int bloc
 template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                        const Vec4<T>* vsn, const uint32_t* offset, Vec4<T>* vout) {
+                        const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
+                        Vec4<T>* vout) {
   const Locator<T> loc = make_locator(ntab, block / lanes, btab, scenes, one);
-  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, vout);
-  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, vout);
+  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, home, vout);
+  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, home, vout);
 }
 
 template <class T>
@@ -908,18 +1021,20 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
                                           const SceneDev<T>*, const SceneDev<T>*,              \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
-                                          Vec4<T>*, Vec4<T>*, int32_t*, uint32_t*, int32_t*,   \
-                                          int2*, uint32_t*, int);                              \
+                                          int32_t*, uint32_t*, int32_t*, int2*, uint32_t*,     \
+                                          int);                                                \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
                                             const SceneDev<T>*, const SceneDev<T>*, int32_t,   \
                                             const int32_t*, const uint32_t*, const int32_t*,   \
                                             const int2*, const uint32_t*, const Vec4<T>*,      \
                                             const Vec4<T>*, const int32_t*,                    \
                                             const TriEntry<T>*, const int32_t*, uint32_t*,     \
-                                            int, Vec4<T>*, Vec4<T>*, int32_t*, int32_t*);      \
+                                            int, const uint8_t*, Vec4<T>*, Vec4<T>*, int32_t*, \
+                                            int32_t*, uint32_t*);                              \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
-                                      const Vec4<T>*, const uint32_t*, Vec4<T>*);              \
+                                      const Vec4<T>*, const uint32_t*, const uint32_t*,        \
+                                      Vec4<T>*);                                               \
   template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,    \
                                        const int32_t*, T*, T*);                                \
   template hipError_t launch_sort_in<T>(hipStream_t, int32_t, Vec4<T>*, Vec4<T>*,               \

@@ -14,7 +14,8 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
-constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the overflow list
+constexpr int kCellIds = 8;
+constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours  // id slots per cell; arrivals beyond go to the overflow list
 // Largest workgroup of the rank/gather kernel: its binary64 form (rank, walls and the mesh contact
 // in double) needs more than the 128 registers a 1024-wide workgroup leaves per lane.
 template <class T>
@@ -31,26 +32,30 @@ constexpr int kPairPad = 4;
 // Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
 // (ntab = workgroups); otherwise btab holds one (scene, slice) entry per workgroup.
 //
-// Integrate (kModeIntegrate: writes the integrated, pre-container state to pos_o/vel_o), hash,
-// count. cell_ids: kCellIds slots per cell (C_pad * kCellIds); ovf: n entries (cell, id);
-// ovf_ctr[0] counts the overflow entries of this pass and must be 0 on entry (k_scan resets it).
+// Hash and count: the cell of each particle's position after the integration step
+// (kModeIntegrate) or of its current position. cell_ids: kCellIds slots per cell
+// (C_pad * kCellIds); ovf: n entries (cell, id); ovf_ctr[0] counts the overflow entries of this
+// pass and must be 0 on entry (k_scan resets it).
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                            const Vec4<T>* vel, const int32_t* id, Vec4<T>* pos_o, Vec4<T>* vel_o,
-                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, int mode);
+                            const Vec4<T>* vel, const int32_t* id, int32_t* cell_of,
+                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
+                            int mode);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
 // (the last entry is the total); clears the counts, moves ovf_ctr[0] to ovf_ctr[1] and zeroes
-// ovf_ctr[0]. tile_state (ntiles words) and ticket persist between launches and are zeroed once at
+// ovf_ctr[0]. tile_state (ntiles words) persists between launches and is zeroed once at
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       unsigned long long* ticket, int32_t ntiles, uint32_t* ovf_ctr,
-                       uint32_t* flags);
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags);
 
-// Stable rank among cell mates, container (kModeContainer: walls or mesh) and non-finite check
-// (kModeCheck) on the state (pos, vel), gather into cell-sorted order.
+// Stable rank among cell mates; the integration step (kModeIntegrate), container (kModeContainer:
+// walls or mesh) and non-finite check (kModeCheck) on the state (pos, vel); gather into
+// cell-sorted order, with the bucket cell (cell_o, optional) and the home word of the gathered
+// position (home_o, for the pair kernel: home cell | kHomeCull when cull[home cell] is set) of
+// every sorted slot. cull: one byte per cell, nonzero where the pair kernel may cull the
+// neighbour cells by distance.
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, int32_t n,
@@ -58,14 +63,16 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
                               const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                               const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
-                              int mode, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o);
+                              int mode, const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o,
+                              int32_t* id_o, int32_t* cell_o, uint32_t* home_o);
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
+// home: the home word of every sorted slot (from the rank/gather pass that produced pos).
 template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                        const Vec4<T>* vsn, const uint32_t* offset, Vec4<T>* vout);
+                        const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
+                        Vec4<T>* vout);
 
 template <class T>
 hipError_t launch_unsort(hipStream_t st, int32_t n, const Vec4<T>* pos, const Vec4<T>* vel,

@@ -193,6 +193,45 @@ void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32) {
   m.off[(size_t)ncell] = (int32_t)m.list.size();
 }
 
+// Cells whose 27 neighbours may be culled by distance in the pair kernel (one byte per cell of the
+// scene's grid). A cell is bounded when every member's snapshot position lies inside its box:
+// interior cells (index 1 .. n-2 on every axis; positions outside the domain are clamped into the
+// boundary cells, and box walls only push particles of boundary cells) with an empty triangle list
+// (a particle pushed by the mesh touched a triangle, which its bucket cell therefore lists). The
+// result marks cells whose 3x3x3 neighbourhood is bounded: the bounded mask eroded along x, y, z.
+std::vector<uint8_t> cull_cells(const HostGrid& g, const MeshHost* m) {
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const size_t nc = (size_t)(nx * ny * nz);
+  std::vector<uint8_t> a(nc, 0), b(nc, 0);
+  auto at = [&](int64_t i, int64_t j, int64_t k) { return (size_t)((i * ny + j) * nz + k); };
+  for (int64_t i = 1; i + 1 < nx; ++i)
+    for (int64_t j = 1; j + 1 < ny; ++j)
+      for (int64_t k = 1; k + 1 < nz; ++k) {
+        const size_t c = at(i, j, k);
+        a[c] = (!m || m->off[c + 1] == m->off[c]) ? 1 : 0;
+      }
+  // erosion: a cell survives when it and both neighbours along the axis are set (none at the edges)
+  auto erode = [&](const std::vector<uint8_t>& src, std::vector<uint8_t>& dst, int axis) {
+    std::fill(dst.begin(), dst.end(), 0);
+    for (int64_t i = 0; i < nx; ++i)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+          const int64_t idx[3] = {i, j, k};
+          const int64_t len = g.n[axis];
+          if (idx[axis] == 0 || idx[axis] == len - 1) continue;
+          int64_t lo[3] = {i, j, k}, hi[3] = {i, j, k};
+          --lo[axis];
+          ++hi[axis];
+          const size_t c = at(i, j, k);
+          dst[c] = src[c] & src[at(lo[0], lo[1], lo[2])] & src[at(hi[0], hi[1], hi[2])];
+        }
+  };
+  erode(a, b, 0);
+  erode(b, a, 1);
+  erode(a, b, 2);
+  return b;
+}
+
 // ---------------------------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------------------------
@@ -235,9 +274,9 @@ struct SimImpl final : SimBase {
   // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
-  DevBuf cell_of, cell_ids, ovf, ovf_ctr;
-  DevBuf count, offset, tile_state, ticket, flags;
-  DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d;
+  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home;
+  DevBuf count, offset, tile_state, flags;
+  DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
   std::map<int, std::unique_ptr<DevBuf>> btab;
@@ -300,6 +339,7 @@ struct SimImpl final : SimBase {
     std::vector<TriBox<T>> tbox;
     std::vector<TriEntry<T>> tent;
     std::vector<int32_t> toff;
+    std::vector<uint8_t> cullh;  // one byte per cell over all scenes
     for (size_t s = 0; s < scenes.size(); ++s) {
       SceneHost& sh = scenes[s];
       double rmax = 0.0;
@@ -334,15 +374,28 @@ struct SimImpl final : SimBase {
       }
       d.grid.h = sh.grid.h;
       d.grid.cell_base = (int32_t)sh.cell_base;
+      for (int a = 0; a < 2; ++a) {  // divisors n_z, n_y (Granlund and Montgomery, fig. 4.1)
+        const uint64_t dv = (uint64_t)sh.grid.n[2 - a];
+        uint32_t l = 0;
+        while ((uint64_t(1) << l) < dv) ++l;
+        const uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << l) - dv)) / dv + 1;
+        d.grid.magic[a] = (uint32_t)m;
+        d.grid.shift[a] = std::min<uint32_t>(l, 1u) | (std::max<uint32_t>(l, 1u) - 1u) << 8;
+      }
       fill_coefficients(d, sh.prm);
       d.rmax = (T)rmax;
       d.p_begin = (int32_t)sh.p_begin;
       d.p_end = (int32_t)(sh.p_begin + sh.n);
       d.mesh = -1;
       d.tcell_base = 0;
+      if (sh.mesh >= 0) build_tri_grid(meshes[(size_t)sh.mesh], sh.grid, rmax, sizeof(T) == 4);
+      {
+        const std::vector<uint8_t> cs =
+            cull_cells(sh.grid, sh.mesh >= 0 ? &meshes[(size_t)sh.mesh] : nullptr);
+        cullh.insert(cullh.end(), cs.begin(), cs.end());
+      }
       if (sh.mesh >= 0) {
         MeshHost& m = meshes[(size_t)sh.mesh];
-        build_tri_grid(m, sh.grid, rmax, sizeof(T) == 4);
         d.mesh = sh.mesh;
         d.tcell_base = (int32_t)toff.size();
         const int32_t tri_base = (int32_t)tgeo.size();
@@ -405,19 +458,20 @@ struct SimImpl final : SimBase {
       id[k].alloc((size_t)nn * sizeof(int32_t));
     }
     cell_of.alloc((size_t)nn * 4);
+    home.alloc((size_t)nn * 4);
     cell_ids.alloc((size_t)cpad * kCellIds * 4);
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1) * 4);
     tile_state.alloc((size_t)ntiles * 8);
-    ticket.alloc(8);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
     scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
     tgeo_d.alloc(std::max<size_t>(tgeo.size(), 1) * sizeof(TriGeo<T>));
     tbox_d.alloc(std::max<size_t>(tbox.size(), 1) * sizeof(TriBox<T>));
     tent_d.alloc((tent.size() + kTriPad) * sizeof(TriEntry<T>));
+    cull_d.alloc((size_t)cpad);
     tcell_off_d.alloc(std::max<size_t>(toff.size(), 1) * 4);
 
     PSX_HIP(hipMemcpy(pos[0].p, p4.data(), (size_t)nn * sizeof(Vec4<T>), hipMemcpyHostToDevice));
@@ -428,7 +482,6 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemset(tile_state.p, 0, tile_state.bytes));
     PSX_HIP(hipMemset(ovf_ctr.p, 0, ovf_ctr.bytes));
     PSX_HIP(hipMemset(cell_ids.p, 0, cell_ids.bytes));
-    PSX_HIP(hipMemset(ticket.p, 0, ticket.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
     PSX_HIP(hipMemcpy(scenes_d.p, sd.data(), sd.size() * sizeof(SceneDev<T>), hipMemcpyHostToDevice));
     scenes_h = sd;
@@ -437,6 +490,8 @@ struct SimImpl final : SimBase {
     if (!tbox.empty())
       PSX_HIP(hipMemcpy(tbox_d.p, tbox.data(), tbox.size() * sizeof(TriBox<T>), hipMemcpyHostToDevice));
     PSX_HIP(hipMemset(tent_d.p, 0, tent_d.bytes));  // the padding entries are read, never used
+    PSX_HIP(hipMemset(cull_d.p, 0, cull_d.bytes));
+    PSX_HIP(hipMemcpy(cull_d.p, cullh.data(), cullh.size(), hipMemcpyHostToDevice));
     if (!tent.empty())
       PSX_HIP(hipMemcpy(tent_d.p, tent.data(), tent.size() * sizeof(TriEntry<T>), hipMemcpyHostToDevice));
     if (!toff.empty())
@@ -460,39 +515,35 @@ struct SimImpl final : SimBase {
     PSX_HIP(e);
   }
 
-  // Bucketing: k_integrate reads (ps, vs), integrates when `mode` says so (writing the
-  // pre-container state to (po, vo), which may alias (ps, vs)), hashes and files the ids `is` into
-  // the cell id blocks; scan into `off`; stable rank, container (kModeContainer) and check
-  // (kModeCheck), gather into (pd, vd, idd). The bucket cell of every sorted slot goes to
-  // cell_sorted when it is not NULL.
-  void bucket(const Vec4<T>* ps, const Vec4<T>* vs, Vec4<T>* po, Vec4<T>* vo, const int32_t* is,
-              Vec4<T>* pd, Vec4<T>* vd, int32_t* idd, uint32_t* off, int32_t* cell_sorted,
-              int mode, Profiler* prof) {
+  // Bucketing: k_integrate hashes the positions of (ps, vs) after the integration step (when `mode`
+  // says so) and files the ids `is` into the cell id blocks; scan into `off`; k_rank_gather ranks,
+  // integrates, applies the container (kModeContainer) and the check (kModeCheck) to (ps, vs) and
+  // gathers into (pd, vd, idd).
+  void bucket(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
+              int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof) {
     int nb_i, nb_s;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
     const int32_t nn = (int32_t)n;
-    const bool integrates = (mode & kModeIntegrate) != 0;
     launch(0, prof, [&] {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
-                                 scenes_d.as<SceneDev<T>>(), one, ps, vs, is, po, vo,
+                                 scenes_d.as<SceneDev<T>>(), one, ps, vs, is,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
                                  mode);
     });
     launch(1, prof, [&] {
-      return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(),
-                         ticket.as<unsigned long long>(), ntiles, ovf_ctr.as<uint32_t>(),
-                         flags.as<uint32_t>());
+      return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
+                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>());
     });
     launch(2, prof, [&] {
       return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
                                    scenes_d.as<SceneDev<T>>(), one, nn, cell_of.as<int32_t>(), off,
                                    cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                   integrates ? po : ps, integrates ? vo : vs, is,
-                                   tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
-                                   flags.as<uint32_t>(), mode, pd, vd, idd, cell_sorted);
+                                   ps, vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
+                                   flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
+                                   cell_sorted, home.as<uint32_t>());
     });
   }
 
@@ -500,7 +551,7 @@ struct SimImpl final : SimBase {
     const int a = cur_pos, vc = cur_vel;
     Vec4<T>* p = pos[a].as<Vec4<T>>();
     Vec4<T>* v = vel[vc].as<Vec4<T>>();
-    bucket(p, v, p, v, id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
+    bucket(p, v, id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
            id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr, mode, prof);
   }
 
@@ -518,7 +569,8 @@ struct SimImpl final : SimBase {
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
                              single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
-                             single() ? scenes_h.data() : nullptr, ps, vs, off, vout);
+                             single() ? scenes_h.data() : nullptr, ps, vs, off,
+                             home.as<uint32_t>(), vout);
     });
   }
 
@@ -538,7 +590,7 @@ struct SimImpl final : SimBase {
   // pre-step state is seg_* for k = 0 and (rs_pos, rs_vnew, rs_id)[k-1] otherwise, so the
   // recompute copies nothing. Both halves are replayed from hipGraphs captured per segment length.
   int64_t win_steps = 0, win_every = 0, win_nseg = 0;
-  DevBuf ck_pos, ck_vel, ck_id, seg_pos, seg_vel, seg_id, tmp_pos, tmp_vel;
+  DevBuf ck_pos, ck_vel, ck_id, seg_pos, seg_vel, seg_id;
   DevBuf rs_pos, rs_vel, rs_id, rs_cell, rs_off, rs_vnew;
   DevBuf posH, velH, idH, offH;
   DevBuf lxA, lvA, lxB, lvB, gk, ge, gg, gout;
@@ -574,7 +626,7 @@ struct SimImpl final : SimBase {
       ck_pos.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
       ck_vel.alloc((size_t)(nseg + 1) * nn * sizeof(Vec4<T>));
       ck_id.alloc((size_t)(nseg + 1) * nn * 4);
-      for (DevBuf* b : {&seg_pos, &seg_vel, &tmp_pos, &tmp_vel, &posH, &velH})
+      for (DevBuf* b : {&seg_pos, &seg_vel, &posH, &velH})
         b->alloc(nn * sizeof(Vec4<T>));
       seg_id.alloc(nn * 4);
       idH.alloc(nn * 4);
@@ -616,7 +668,7 @@ struct SimImpl final : SimBase {
       Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
       uint32_t* so = rs_off.as<uint32_t>() + k * nc;
-      bucket(pre_pos(k), pre_vel(k), tmp_pos.as<Vec4<T>>(), tmp_vel.as<Vec4<T>>(), pre_id(k), sp, sv,
+      bucket(pre_pos(k), pre_vel(k), pre_id(k), sp, sv,
              rs_id.as<int32_t>() + k * nn, so, rs_cell.as<int32_t>() + k * nn,
              kModeIntegrate | kModeContainer | kModeCheck, nullptr);
       pairs(sp, sv, so, rs_vnew.as<Vec4<T>>() + k * nn, nullptr);
@@ -632,7 +684,7 @@ struct SimImpl final : SimBase {
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
       const int32_t* si = rs_id.as<int32_t>() + k * nn;
       // home-cell bucketing of the snapshot (hash only: nothing is written to sp/sv)
-      bucket(sp, sv, sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+      bucket(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
              offH.as<uint32_t>(), nullptr, 0, nullptr);
       PSX_HIP(launch_adj_pairs<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
                                   rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,

@@ -104,7 +104,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    # an explicit path, else $PSX_LIBRARY (as the CLI honours it), else the in-tree build
+    p = Path(path) if path else Path(os.environ.get("PSX_LIBRARY") or LIB_PATH)
     if not p.exists():
         raise RuntimeError(
             f"HIP backend not built: {p} is missing (run `make -C particle-stabilizer_amd` "
