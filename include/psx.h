@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PSX_ABI_VERSION 1
+#define PSX_ABI_VERSION 2
 
 enum psx_status {
   PSX_OK = 0,
@@ -79,6 +79,9 @@ typedef struct psx_tuning {
   int32_t block_pairs;      /* workgroup size of the pair-contact kernel */
   int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
   int32_t pair_variant;     /* lanes per particle in the pair-contact kernel: 0 or 1, 2, 4, 8, 16 */
+  int32_t sort_variant;     /* cell lists of the bucketing: 1 per-cell id blocks (four kernels per
+                               step), 2 dense lists filled by a scatter kernel (five), 0 automatic
+                               (id blocks up to 2^18 cells) */
 } psx_tuning;
 
 typedef struct psx_sim psx_sim;

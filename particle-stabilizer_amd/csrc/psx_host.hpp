@@ -12,7 +12,7 @@
 
 namespace psx {
 
-constexpr int kNumKernels = 4;
+constexpr int kNumKernels = 5;
 
 int set_error(int code, const std::string& msg);
 
@@ -50,6 +50,7 @@ inline psx_tuning default_tuning() {
   t.block_pairs = 256;
   t.use_graph = 0;
   t.pair_variant = 0;
+  t.sort_variant = 0;
   return t;
 }
 

@@ -305,13 +305,13 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
 // ---------------------------------------------------------------------------------------------
 // K1: integrate + cell hash + container + counts
 // ---------------------------------------------------------------------------------------------
-template <class T, bool kTable>
+template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(1024)
     k_integrate(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
                 const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
-                uint32_t* __restrict__ ovf_ctr, int mode) {
+                uint32_t* __restrict__ ovf_ctr, uint32_t* __restrict__ slot_of, int mode) {
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     if (p >= p1) return;
@@ -329,19 +329,35 @@ __global__ void __launch_bounds__(1024)
     const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
     cell_of[p] = c;
     const uint32_t slot = atomicAdd(count + c, 1u);
-    if (slot < (uint32_t)kCellIds) {
-      cell_ids[(size_t)c * kCellIds + slot] = me;
-    } else {  // at most n entries in all
-      const uint32_t o = atomicAdd(ovf_ctr, 1u);
-      ovf[o] = make_int2(c, me);
+    if constexpr (kBlocks) {
+      if (slot < (uint32_t)kCellIds) {
+        cell_ids[(size_t)c * kCellIds + slot] = me;
+      } else {  // at most n entries in all
+        const uint32_t o = atomicAdd(ovf_ctr, 1u);
+        ovf[o] = make_int2(c, me);
+      }
+    } else {
+      slot_of[p] = slot;  // the dense list is filled once the offsets are known
     }
   });
 }
 
+// Dense cell lists (the scatter form of the bucketing): id to offset[cell] + arrival slot, an
+// unordered list per cell that the rank/gather kernel orders by id.
+__global__ void __launch_bounds__(1024)
+    k_scatter_ids(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
+                  const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ offset,
+                  const int32_t* __restrict__ id, int32_t* __restrict__ list) {
+  const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
+  if (p >= n) return;
+  list[offset[cell_of[p]] + slot_of[p]] = id[p];
+}
+
 // ---------------------------------------------------------------------------------------------
 // K3: exclusive scan of the per-cell counts in one pass (decoupled look-back). Each workgroup
-// scans one tile of kScanTile cells: 16 cells per thread summed serially, wave64 inclusive scans
-// of the thread totals, LDS wave totals. It publishes the tile total, looks back over the
+// scans one tile of kScanTile cells as four sub-tiles of 1024 (coalesced uint4 loads and stores,
+// a thread's four cells summed, wave64 inclusive scans, LDS wave totals of all four sub-tiles
+// behind one barrier). It publishes the tile total, looks back over the
 // published totals of the tiles before it (64 at a time, one per lane of wave 0) until it meets a
 // published inclusive prefix, publishes its own inclusive prefix, writes the offsets and clears
 // the counts for the next step.
@@ -370,7 +386,11 @@ __global__ void __launch_bounds__(kScanThreads)
     k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
            uint64_t* __restrict__ tile_state, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
            uint32_t* __restrict__ flags) {
-  __shared__ uint32_t wsum[kScanThreads / 64];
+  // A tile is kScanRounds sub-tiles of kScanThreads uint4 (4 cells each): thread t holds uint4
+  // t of every sub-tile, so every load and store of a wave is contiguous.
+  constexpr int kScanRounds = kScanPerThread / 4;
+  constexpr int kWaves = kScanThreads / 64;
+  __shared__ uint32_t wsum[kScanRounds][kWaves];
   __shared__ uint32_t s_prefix;
   const int32_t tile = (int32_t)blockIdx.x;
   // this tile's word from the previous launch: its epoch + 1 is this launch's (read alongside the
@@ -384,28 +404,25 @@ __global__ void __launch_bounds__(kScanThreads)
     ovf_ctr[1] = ovf_ctr[0];
     ovf_ctr[0] = 0u;
   }
-  const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanPerThread;
-  uint4* __restrict__ src = reinterpret_cast<uint4*>(count + base);
-  uint32_t v[kScanPerThread];
-#pragma unroll
-  for (int q = 0; q < kScanPerThread / 4; ++q) {
-    const uint4 u = src[q];
-    v[4 * q + 0] = u.x;
-    v[4 * q + 1] = u.y;
-    v[4 * q + 2] = u.z;
-    v[4 * q + 3] = u.w;
-  }
-  uint32_t tot = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) tot += v[k];
-  const uint32_t wi = wave_inclusive_scan(tot);
+  uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kScanTile);
+  uint4 u[kScanRounds];
+  uint32_t tot[kScanRounds], wi[kScanRounds];
   const int w = threadIdx.x >> 6;
   const int lane = lane_id();
-  if (lane == 63) wsum[w] = wi;
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) u[q] = src[q * kScanThreads + threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) {
+    tot[q] = (u[q].x + u[q].y) + (u[q].z + u[q].w);
+    wi[q] = wave_inclusive_scan(tot[q]);
+    if (lane == 63) wsum[q][w] = wi[q];
+  }
   __syncthreads();
   uint32_t agg = 0;
 #pragma unroll
-  for (int q = 0; q < kScanThreads / 64; ++q) agg += wsum[q];
+  for (int q = 0; q < kScanRounds; ++q)
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) agg += wsum[q][k];
 
   if (w == 0) {
     if (lane == 0)
@@ -447,25 +464,32 @@ __global__ void __launch_bounds__(kScanThreads)
     }
   }
   __syncthreads();
-  uint32_t run = s_prefix + wi - tot;
-  for (int q = 0; q < w; ++q) run += wsum[q];
+  uint4* __restrict__ dst = reinterpret_cast<uint4*>(offset + (size_t)tile * kScanTile);
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  uint32_t round_base = s_prefix;  // cells of the sub-tiles before round q
 #pragma unroll
-  for (int q = 0; q < kScanPerThread / 4; ++q) {
+  for (int q = 0; q < kScanRounds; ++q) {
+    uint32_t run = round_base + wi[q] - tot[q];
+    uint32_t rtot = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      run += (k < w) ? wsum[q][k] : 0u;
+      rtot += wsum[q][k];
+    }
     uint4 o;
     o.x = run;
-    run += v[4 * q + 0];
+    run += u[q].x;
     o.y = run;
-    run += v[4 * q + 1];
+    run += u[q].y;
     o.z = run;
-    run += v[4 * q + 2];
+    run += u[q].z;
     o.w = run;
-    run += v[4 * q + 3];
-    reinterpret_cast<uint4*>(offset + base)[q] = o;
-    src[q] = zero;
+    dst[q * kScanThreads + threadIdx.x] = o;
+    if ((u[q].x | u[q].y | u[q].z | u[q].w) != 0u) src[q * kScanThreads + threadIdx.x] = zero;
+    round_base += rtot;
   }
-  if (tile == ntiles - 1 && threadIdx.x == kScanThreads - 1)
-    offset[(size_t)ntiles * kScanTile] = run;  // total: the offset one past the last cell
+  if (tile == ntiles - 1 && threadIdx.x == 0)
+    offset[(size_t)ntiles * kScanTile] = round_base;  // total: the offset one past the last cell
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -475,12 +499,13 @@ __global__ void __launch_bounds__(kScanThreads)
 // slots, the first count of which are this pass's) and, for a cell with more than kCellIds
 // members, from the overflow list.
 // ---------------------------------------------------------------------------------------------
-template <class T, bool kTable>
+template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(kMaxBlockSort<T>)
     k_rank_gather(const Locator<T> loc, const int32_t* __restrict__ cell_of,
                   const uint32_t* __restrict__ offset, const int32_t* __restrict__ cell_ids,
                   const int2* __restrict__ ovf, const uint32_t* __restrict__ ovf_ctr,
-                  const Vec4<T>* __restrict__ pos, const Vec4<T>* __restrict__ vel,
+                  const int32_t* __restrict__ list, const Vec4<T>* __restrict__ pos,
+                  const Vec4<T>* __restrict__ vel,
                   const int32_t* __restrict__ id, const TriEntry<T>* __restrict__ tent,
                   const int32_t* __restrict__ tcell_off, uint32_t* __restrict__ flags, int mode,
                   const uint8_t* __restrict__ cull, int32_t n, Vec4<T>* __restrict__ pos_o,
@@ -513,19 +538,23 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       }
       cull_c = cull[c];
       const uint32_t b = offset[c], e = offset[c + 1];
-      const uint32_t cnt = e - b;
-      const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
-      const int4 m0 = blk[0], m1 = blk[1];
-      const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
       uint32_t rank = 0;
+      if constexpr (kBlocks) {
+        const uint32_t cnt = e - b;
+        const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
+        const int4 m0 = blk[0], m1 = blk[1];
+        const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
 #pragma unroll
-      for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
-      if (cnt > (uint32_t)kCellIds) {
-        const uint32_t len = ovf_ctr[1];
-        for (uint32_t o = 0; o < len; ++o) {
-          const int2 q = ovf[o];
-          rank += (q.x == c && q.y < me) ? 1u : 0u;
+        for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
+        if (cnt > (uint32_t)kCellIds) {
+          const uint32_t len = ovf_ctr[1];
+          for (uint32_t o = 0; o < len; ++o) {
+            const int2 q = ovf[o];
+            rank += (q.x == c && q.y < me) ? 1u : 0u;
+          }
         }
+      } else {
+        for (uint32_t k = b; k < e; ++k) rank += (list[k] < me) ? 1u : 0u;
       }
       // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
       // arrays even if a stalled scan (reported through the flags) left them short
@@ -920,18 +949,32 @@ static Locator<T> make_locator(int ntab, int per_block, const BlockInfo* btab,
 
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                            const Vec4<T>* vel, const int32_t* id, int32_t* cell_of,
-                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
-                            int mode) {
+                            const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                            const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
+                            uint32_t* ovf_ctr, uint32_t* slot_of, int mode) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
+  const dim3 grid(xcd_grid(ntab)), blk(block);
+#define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
+  PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
+             count, cell_ids, ovf, ovf_ctr, slot_of, mode)
   if (one) {
-    PSX_LAUNCH((k_integrate<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
-               id, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+    if (blocks) PSX_INTEGRATE(false, true);
+    else PSX_INTEGRATE(false, false);
   } else {
-    PSX_LAUNCH((k_integrate<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, pos, vel,
-               id, cell_of, count, cell_ids, ovf, ovf_ctr, mode);
+    if (blocks) PSX_INTEGRATE(true, true);
+    else PSX_INTEGRATE(true, false);
   }
+#undef PSX_INTEGRATE
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
+                              const uint32_t* slot_of, const uint32_t* offset, const int32_t* id,
+                              int32_t* list) {
+  const int32_t tiles = (int32_t)((n + block - 1) / block);
+  PSX_LAUNCH(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of, slot_of,
+             offset, id, list);
   return hipGetLastError();
 }
 
@@ -944,23 +987,28 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                              const SceneDev<T>* scenes, const SceneDev<T>* one, int32_t n,
-                              const int32_t* cell_of, const uint32_t* offset,
+                              const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                              int32_t n, const int32_t* cell_of, const uint32_t* offset,
                               const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
-                              const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
-                              const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
-                              int mode, const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o,
-                              int32_t* id_o, int32_t* cell_o, uint32_t* home_o) {
+                              const int32_t* list, const Vec4<T>* pos, const Vec4<T>* vel,
+                              const int32_t* id, const TriEntry<T>* tent,
+                              const int32_t* tcell_off, uint32_t* flags, int mode,
+                              const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+                              int32_t* cell_o, uint32_t* home_o) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
+  const dim3 grid(xcd_grid(ntab)), blk(block);
+#define PSX_RANK(TABLE, BLOCKS)                                                                  \
+  PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
+             cell_ids, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode, cull, n,  \
+             pos_o, vel_o, id_o, cell_o, home_o)
   if (one) {
-    PSX_LAUNCH((k_rank_gather<T, false>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
-               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, cull,
-               n, pos_o, vel_o, id_o, cell_o, home_o);
+    if (blocks) PSX_RANK(false, true);
+    else PSX_RANK(false, false);
   } else {
-    PSX_LAUNCH((k_rank_gather<T, true>), dim3(xcd_grid(ntab)), dim3(block), 0, st, loc, cell_of,
-               offset, cell_ids, ovf, ovf_ctr, pos, vel, id, tent, tcell_off, flags, mode, cull,
-               n, pos_o, vel_o, id_o, cell_o, home_o);
+    if (blocks) PSX_RANK(true, true);
+    else PSX_RANK(true, false);
   }
+#undef PSX_RANK
   return hipGetLastError();
 }
 
@@ -1019,18 +1067,19 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
 
 #define PSX_INSTANTIATE(T)                                                                      \
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
-                                          const SceneDev<T>*, const SceneDev<T>*,              \
+                                          const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, int2*, uint32_t*,     \
-                                          int);                                                \
+                                          uint32_t*, int);                                     \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
-                                            const SceneDev<T>*, const SceneDev<T>*, int32_t,   \
-                                            const int32_t*, const uint32_t*, const int32_t*,   \
-                                            const int2*, const uint32_t*, const Vec4<T>*,      \
-                                            const Vec4<T>*, const int32_t*,                    \
-                                            const TriEntry<T>*, const int32_t*, uint32_t*,     \
-                                            int, const uint8_t*, Vec4<T>*, Vec4<T>*, int32_t*, \
-                                            int32_t*, uint32_t*);                              \
+                                            const SceneDev<T>*, const SceneDev<T>*, bool,      \
+                                            int32_t, const int32_t*, const uint32_t*,          \
+                                            const int32_t*, const int2*, const uint32_t*,      \
+                                            const int32_t*, const Vec4<T>*, const Vec4<T>*,    \
+                                            const int32_t*, const TriEntry<T>*,                \
+                                            const int32_t*, uint32_t*, int, const uint8_t*,    \
+                                            Vec4<T>*, Vec4<T>*, int32_t*, int32_t*,            \
+                                            uint32_t*);                                        \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \

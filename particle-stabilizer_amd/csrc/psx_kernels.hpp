@@ -33,15 +33,22 @@ constexpr int kPairPad = 4;
 // (ntab = workgroups); otherwise btab holds one (scene, slice) entry per workgroup.
 //
 // Hash and count: the cell of each particle's position after the integration step
-// (kModeIntegrate) or of its current position. cell_ids: kCellIds slots per cell
-// (C_pad * kCellIds); ovf: n entries (cell, id); ovf_ctr[0] counts the overflow entries of this
-// pass and must be 0 on entry (k_scan resets it).
+// (kModeIntegrate) or of its current position, and its arrival slot in the cell (atomic count).
+// blocks: the id is filed into the cell's id block (cell_ids: kCellIds slots per cell,
+// C_pad * kCellIds; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them
+// and must be 0 on entry, k_scan resets it). Otherwise the slot goes to slot_of and
+// launch_scatter_ids builds dense cell lists once the offsets are known.
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                            const Vec4<T>* vel, const int32_t* id, int32_t* cell_of,
-                            uint32_t* count, int32_t* cell_ids, int2* ovf, uint32_t* ovf_ctr,
-                            int mode);
+                            const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                            const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
+                            uint32_t* ovf_ctr, uint32_t* slot_of, int mode);
+
+// list[offset[cell_of[p]] + slot_of[p]] = id[p]: unordered dense cell lists.
+hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
+                              const uint32_t* slot_of, const uint32_t* offset, const int32_t* id,
+                              int32_t* list);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
 // (the last entry is the total); clears the counts, moves ovf_ctr[0] to ovf_ctr[1] and zeroes
@@ -50,7 +57,8 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags);
 
-// Stable rank among cell mates; the integration step (kModeIntegrate), container (kModeContainer:
+// Stable rank among cell mates (from the id blocks, or from the dense lists when !blocks); the
+// integration step (kModeIntegrate), container (kModeContainer:
 // walls or mesh) and non-finite check (kModeCheck) on the state (pos, vel); gather into
 // cell-sorted order, with the bucket cell (cell_o, optional) and the home word of the gathered
 // position (home_o, for the pair kernel: home cell | kHomeCull when cull[home cell] is set) of
@@ -58,13 +66,14 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 // neighbour cells by distance.
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
-                              const SceneDev<T>* scenes, const SceneDev<T>* one, int32_t n,
-                              const int32_t* cell_of, const uint32_t* offset,
+                              const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                              int32_t n, const int32_t* cell_of, const uint32_t* offset,
                               const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
-                              const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
-                              const TriEntry<T>* tent, const int32_t* tcell_off, uint32_t* flags,
-                              int mode, const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o,
-                              int32_t* id_o, int32_t* cell_o, uint32_t* home_o);
+                              const int32_t* list, const Vec4<T>* pos, const Vec4<T>* vel,
+                              const int32_t* id, const TriEntry<T>* tent,
+                              const int32_t* tcell_off, uint32_t* flags, int mode,
+                              const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
+                              int32_t* cell_o, uint32_t* home_o);
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
 // home: the home word of every sorted slot (from the rank/gather pass that produced pos).

@@ -264,7 +264,7 @@ struct DevBuf {
 };
 
 static const char* kKernelNames[kNumKernels] = {"k_integrate", "k_scan", "k_rank_gather",
-                                                "k_pairs"};
+                                                "k_pairs", "k_scatter_ids"};
 
 // ---------------------------------------------------------------------------------------------
 // the simulation
@@ -274,7 +274,7 @@ struct SimImpl final : SimBase {
   // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
-  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home;
+  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, slot_of, list;
   DevBuf count, offset, tile_state, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
@@ -459,6 +459,8 @@ struct SimImpl final : SimBase {
     }
     cell_of.alloc((size_t)nn * 4);
     home.alloc((size_t)nn * 4);
+    slot_of.alloc((size_t)nn * 4);
+    list.alloc((size_t)nn * 4);
     cell_ids.alloc((size_t)cpad * kCellIds * 4);
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
@@ -519,6 +521,16 @@ struct SimImpl final : SimBase {
   // says so) and files the ids `is` into the cell id blocks; scan into `off`; k_rank_gather ranks,
   // integrates, applies the container (kModeContainer) and the check (kModeCheck) to (ps, vs) and
   // gathers into (pd, vd, idd).
+  // Cell lists of the bucketing: per-cell id blocks (one kernel fewer per step, but every id lands
+  // in its own sparse 32-byte block) or dense lists (a scatter kernel, contiguous lists). On MI355X
+  // the id blocks pay off only on small grids, where the launch they save is a visible share of
+  // the step (measured: equal at 4e5 cells, 25-30 % slower from 1e7 cells).
+  bool id_blocks() const {
+    if (tuning.sort_variant == 1) return true;
+    if (tuning.sort_variant == 2) return false;
+    return cpad <= (int64_t(1) << 18);
+  }
+
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
               int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof) {
     int nb_i, nb_s;
@@ -526,22 +538,29 @@ struct SimImpl final : SimBase {
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
     const int32_t nn = (int32_t)n;
+    const bool blocks = id_blocks();
     launch(0, prof, [&] {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
-                                 scenes_d.as<SceneDev<T>>(), one, ps, vs, is,
+                                 scenes_d.as<SceneDev<T>>(), one, blocks, ps, vs, is,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                 mode);
+                                 slot_of.as<uint32_t>(), mode);
     });
     launch(1, prof, [&] {
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
                          ovf_ctr.as<uint32_t>(), flags.as<uint32_t>());
     });
+    if (!blocks)
+      launch(4, prof, [&] {
+        return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
+                                  slot_of.as<uint32_t>(), off, is, list.as<int32_t>());
+      });
     launch(2, prof, [&] {
       return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
-                                   scenes_d.as<SceneDev<T>>(), one, nn, cell_of.as<int32_t>(), off,
-                                   cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                   ps, vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
+                                   scenes_d.as<SceneDev<T>>(), one, blocks, nn,
+                                   cell_of.as<int32_t>(), off, cell_ids.as<int32_t>(),
+                                   ovf.as<int2>(), ovf_ctr.as<uint32_t>(), list.as<int32_t>(), ps,
+                                   vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
                                    cell_sorted, home.as<uint32_t>());
     });
@@ -1170,6 +1189,8 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
   if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
   if (sim->impl->dtype() == PSX_F64 && c.block_sort > psx::kMaxBlockSort<double>)
     return set_error(PSX_ERR_INVALID, "block_sort must be at most 512 for a binary64 simulation");
+  if (c.sort_variant < 0 || c.sort_variant > 2)
+    return set_error(PSX_ERR_INVALID, "sort_variant must be 0 (automatic), 1 (id blocks) or 2 (dense lists)");
   if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
       c.pair_variant != 8 && c.pair_variant != 16)
     return set_error(PSX_ERR_INVALID, "pair_variant (lanes per particle) must be 0, 1, 2, 4, 8 or 16");

@@ -7,7 +7,9 @@ each kernel keeps the shape with the smallest median time over its rounds. Tuned
 
 - block_integrate: workgroup size of the integrate / hash / container kernel;
 - block_sort: workgroup size of the rank / gather / container kernel (at most 512 in binary64);
-- (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel.
+- (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel;
+- sort_variant: per-cell id blocks or dense lists for the bucketing (half of the rounds each; the
+  variant with the smaller hash + scan + scatter + gather time wins).
 
 The cell size stays what the reference's grid rule fixes (it changes the neighbour search, not
 only the speed), and every launch shape gives bit-identical results, so tuning never changes a
@@ -57,35 +59,58 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
                     "cached": True}
 
     p0, v0 = sim.state()
-    best: dict[str, tuple[float, object]] = {}
-    n_rounds = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
-    samples: dict[tuple[str, object], list] = {}
+    n_half = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
+    samples: dict[tuple, list] = {}
     # the binary64 rank/gather kernel takes workgroups of at most 512
     sort_blocks = BLOCKS if np.dtype(sim.dtype) == np.float32 else tuple(b for b in BLOCKS if b <= 512)
-    for k in range(n_rounds):
+    for k in range(2 * n_half):  # first half per-cell id blocks, second half dense lists
+        sv = 1 if k < n_half else 2
         bi = BLOCKS[k % len(BLOCKS)]
         bs = sort_blocks[(k // len(BLOCKS)) % len(sort_blocks)]
         bp, g = PAIR_SHAPES[k % len(PAIR_SHAPES)]
         sim.set_tuning(block_integrate=bi, block_sort=bs, block_pairs=bp, pair_variant=g,
-                       use_graph=0)
+                       sort_variant=sv, use_graph=0)
         prof = sim.profile(steps)
         for name, d in prof.items():
-            shape = {"k_integrate": bi, "k_rank_gather": bs, "k_pairs": (bp, g)}.get(name)
-            if shape is None:
+            shape = {"k_integrate": (bi, sv), "k_scan": (0, sv), "k_rank_gather": (bs, sv),
+                     "k_scatter_ids": (bs, sv), "k_pairs": (bp, g)}.get(name)
+            if shape is None or d.get("per_step", 1) == 0:
                 continue
-            samples.setdefault((name, shape), []).append(d["ms"])
-    for (name, shape), ms in samples.items():
-        t = sorted(ms)[len(ms) // 2]  # median over the rounds that used this shape
-        if name not in best or t < best[name][0]:
-            best[name] = (t, shape)
-    bs_best = best["k_rank_gather"][1]
-    bp_best, g_best = best["k_pairs"][1]
-    tuning = {"block_integrate": int(best["k_integrate"][1]), "block_sort": int(bs_best),
+            samples.setdefault((name,) + shape, []).append(d["ms"])
+
+    def med(name, *key):
+        ms = samples.get((name,) + key)
+        return sorted(ms)[len(ms) // 2] if ms else None
+
+    # per bucketing variant: fastest hash kernel + fastest gather (+ scatter) size + scan
+    plans = {}
+    for sv in (1, 2):
+        bi_ms = {bi: med("k_integrate", bi, sv) for bi in BLOCKS}
+        bs_ms = {}
+        for bs in sort_blocks:
+            r = med("k_rank_gather", bs, sv)
+            sc = med("k_scatter_ids", bs, sv) if sv == 2 else 0.0
+            if r is not None and sc is not None:
+                bs_ms[bs] = r + sc
+        bi_ms = {b: t for b, t in bi_ms.items() if t is not None}
+        scan = med("k_scan", 0, sv) or 0.0
+        if bi_ms and bs_ms:
+            bi = min(bi_ms, key=bi_ms.get)
+            bs = min(bs_ms, key=bs_ms.get)
+            plans[sv] = (bi_ms[bi] + bs_ms[bs] + scan, bi, bs)
+    sv_best = min(plans, key=lambda v: plans[v][0])
+    _, bi_best, bs_best = plans[sv_best]
+    pair_ms = {(bp, g): med("k_pairs", bp, g) for bp, g in PAIR_SHAPES}
+    pair_ms = {k: t for k, t in pair_ms.items() if t is not None}
+    bp_best, g_best = min(pair_ms, key=pair_ms.get)
+    tuning = {"block_integrate": int(bi_best), "block_sort": int(bs_best),
               "block_pairs": int(bp_best), "pair_variant": int(g_best),
-              "use_graph": 1 if use_graph else 0}
+              "sort_variant": int(sv_best), "use_graph": 1 if use_graph else 0}
     sim.set_tuning(**tuning)
     sim.set_state(p0, v0)
-    kernel_ms = {k: best[k][0] for k in ("k_integrate", "k_rank_gather", "k_pairs")}
+    kernel_ms = {"bucketing_id_blocks": plans.get(1, (None,))[0],
+                 "bucketing_dense_lists": plans.get(2, (None,))[0],
+                 "k_pairs": pair_ms[(bp_best, g_best)]}
     if path is not None:
         data = _load(path)
         data[key] = {"tuning": tuning, "kernel_ms": kernel_ms}

@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert sorted(set(names)) == sorted(set(_lib.exported_symbols()))
-    assert lib.psx_abi_version() == 1
+    version = int(re.search(r"#define PSX_ABI_VERSION (\d+)", HEADER.read_text()).group(1))
+    assert lib.psx_abi_version() == version
 
 
 def test_struct_layout_matches_c(tmp_path):
@@ -98,5 +99,6 @@ def test_null_handles_rejected():
     assert lib.psx_num_particles(None) == -1
     assert lib.psx_kernel_name(0) == b"k_integrate"
     assert lib.psx_kernel_name(3) == b"k_pairs"
-    assert lib.psx_kernel_name(4) is None
+    assert lib.psx_kernel_name(4) == b"k_scatter_ids"
+    assert lib.psx_kernel_name(5) is None
     assert lib.psx_kernel_name(99) is None

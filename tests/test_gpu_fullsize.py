@@ -10,14 +10,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _bucket_check(psx_mod, oracle, sc):
-    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
-        got = s.bucketing()
-        nc = s.num_cells
     v = sc.mesh.reshape(-1, 3)
     rmax = float(np.float32(sc.radius).max())
     ref = oracle.bucketing(rmax, v.min(0), v.max(0), np.float32(sc.pos), dtype="f32")
-    for k in ("cell", "count", "offset", "perm"):
-        assert_bitwise(got[k], ref[k], k)
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
+        for variant in (1, 2):  # per-cell id blocks, dense lists
+            s.set_tuning(sort_variant=variant)
+            got = s.bucketing()
+            for k in ("cell", "count", "offset", "perm"):
+                assert_bitwise(got[k], ref[k], f"{k} sort_variant={variant}")
+        nc = s.num_cells
     assert got["offset"][-1] == sc.radius.shape[0] and nc == got["count"].shape[0]
     return got
 

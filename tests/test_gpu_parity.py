@@ -13,10 +13,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _bucket_compare(oracle, sim, pos, dtype, rmax, mn, mx):
-    got = sim.bucketing()
     ref = oracle.bucketing(rmax, mn, mx, as_dtype(pos, dtype), dtype=dtype)
-    for k in ("cell", "count", "offset", "perm"):
-        assert_bitwise(got[k], ref[k], k)
+    for variant in (1, 2):  # per-cell id blocks, dense lists
+        sim.set_tuning(sort_variant=variant)
+        got = sim.bucketing()
+        for k in ("cell", "count", "offset", "perm"):
+            assert_bitwise(got[k], ref[k], f"{k} sort_variant={variant}")
     g = sim.grid()
     assert g["h"] == ref["h"] and np.array_equal(g["n"], ref["n"])
 
@@ -85,16 +87,18 @@ def test_bucketing_cell_overflow(psx_mod, oracle, dtype):
     from psx.sim import Scene
     sc = Scene(pos, vel, r, prm, mass_of(r))
     ref_p, ref_v = oracle_run(oracle, sc, 3)
-    p, v = hip_run(sc, 3)
-    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
-    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+    for variant in (1, 2):  # id blocks with their overflow list, dense lists
+        p, v = hip_run(sc, 3, sort_variant=variant)
+        assert_bitwise(as_dtype(p, dtype), ref_p, f"pos sort_variant={variant}")
+        assert_bitwise(as_dtype(v, dtype), ref_v, f"vel sort_variant={variant}")
 
 
 VARIANTS = [dict(block_pairs=256, block_integrate=256),
-            dict(block_pairs=128, block_integrate=64, pair_variant=2),
-            dict(block_pairs=64, block_integrate=1024, block_sort=64, pair_variant=4),
-            dict(block_pairs=1024, block_integrate=128, block_sort=1024, pair_variant=8),
-            dict(block_pairs=512, pair_variant=16)]
+            dict(block_pairs=128, block_integrate=64, pair_variant=2, sort_variant=2),
+            dict(block_pairs=64, block_integrate=1024, block_sort=64, pair_variant=4, sort_variant=1),
+            dict(block_pairs=1024, block_integrate=128, block_sort=1024, pair_variant=8,
+                 sort_variant=2),
+            dict(block_pairs=512, pair_variant=16, sort_variant=1)]
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -165,6 +169,9 @@ def test_tuning_rejects_bad_lanes(psx_mod):
         with pytest.raises(psx_mod.PsxError, match="512"):
             s.set_tuning(block_sort=1024)  # binary64 rank/gather: at most 512
         s.set_tuning(block_sort=512)
+        for bad in (-1, 3):
+            with pytest.raises(psx_mod.PsxError, match="sort_variant"):
+                s.set_tuning(sort_variant=bad)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])

@@ -17,7 +17,7 @@ class FakeSim:
         self.num_cells = 4096
         self.num_scenes = 1
         self.tuning = {"block_integrate": 256, "block_sort": 256, "block_pairs": 256,
-                       "use_graph": 0, "pair_variant": 0}
+                       "use_graph": 0, "pair_variant": 0, "sort_variant": 0}
         self.pos = np.arange(3000.0).reshape(1000, 3)
         self.vel = -self.pos
         self.profiled = 0
@@ -36,25 +36,30 @@ class FakeSim:
         self.profiled += 1
         t = self.tuning
         g = max(1, t["pair_variant"])
+        dense = t["sort_variant"] == 2
         return {
-            "k_integrate": {"ms": 0.01 + abs(t["block_integrate"] - 512) * 1e-5},
-            "k_scan": {"ms": 0.004},
-            "k_rank_gather": {"ms": 0.005 + abs(t["block_sort"] - 128) * 1e-6},
-            "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6},
+            "k_integrate": {"ms": 0.01 + abs(t["block_integrate"] - 512) * 1e-5 - 0.004 * dense,
+                            "per_step": 1},
+            "k_scan": {"ms": 0.004, "per_step": 1},
+            "k_rank_gather": {"ms": 0.005 + abs(t["block_sort"] - 128) * 1e-6, "per_step": 1},
+            "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6,
+                        "per_step": 1},
+            "k_scatter_ids": {"ms": 0.003 if dense else 0.0, "per_step": 1 if dense else 0},
         }
 
 
 def test_picks_fastest_shapes_and_restores_state():
     sim = FakeSim()
     out = tune.autotune(sim, steps=2)
+    # dense lists: the hash kernel saves 0.004 ms and the scatter kernel costs 0.003 ms
     assert out["tuning"] == {"block_integrate": 512, "block_sort": 128, "block_pairs": 256,
-                             "pair_variant": 4, "use_graph": 1}
+                             "pair_variant": 4, "sort_variant": 2, "use_graph": 1}
     assert sim.tuning["pair_variant"] == 4 and sim.tuning["use_graph"] == 1
     assert sim.restored is not None
     np.testing.assert_array_equal(sim.restored[0], sim.pos)
     np.testing.assert_array_equal(sim.restored[1], sim.vel)
     # every pair shape and every block size was timed
-    assert sim.profiled == len(tune.PAIR_SHAPES) * 2
+    assert sim.profiled == len(tune.PAIR_SHAPES) * 2 * 2
 
 
 def test_cache_written_and_reused(tmp_path):
