@@ -80,42 +80,53 @@ __device__ inline void column_slices(const GridDev& g, int32_t hx, int32_t hy, i
 
 }  // namespace
 
-// Pair adjoint. Thread per particle of the bucket-sorted snapshot (posS, velS, idS, offS, cellS =
-// bucket cell per slot); posH/velH/idH/offH is the same snapshot sorted by home cell. lx/lv: adjoint
-// of the post-step state (original order, 3 T per particle); writes the adjoint of (x'', v'') to
-// lx_o/lv_o and adds the parameter partials to gk/ge (original order).
-template <class T>
+// Adjoint of one step, one group of G lanes per particle of the bucket-sorted snapshot (posS,
+// velS, idS, offS, cellS = bucket cell per slot; xpreS = the integrated, pre-container position
+// per slot); posH/velH/idH/offH is the same snapshot sorted by home cell. lx/lv: adjoint of the
+// post-step state (original order, 3 T per particle); writes the adjoint of the pre-step state to
+// lx_o/lv_o and adds the parameter partials to gk/ge/gg (original order).
+//
+// Pair adjoint: the lanes of a group take every G-th candidate of each column slice, for i's own
+// kicks (A) and for the kicks i contributes to (B); their partial sums are combined by a fixed
+// xor butterfly, so the result is the same on every run and on every lane. Lane 0 then applies the
+// container adjoint (box walls: J = diag; mesh: (I - n n^T) in reverse hit order for positions,
+// (e / cnt) sum (I - 2 n n^T) for velocities) and the integrate adjoint.
+template <class T, int G>
 __global__ void __launch_bounds__(256)
-    k_adj_pairs(int32_t ntab, const BlockInfo* __restrict__ btab,
-                const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ posS,
-                const Vec4<T>* __restrict__ velS, const int32_t* __restrict__ idS,
-                const uint32_t* __restrict__ offS, const int32_t* __restrict__ cellS,
-                const Vec4<T>* __restrict__ posH, const Vec4<T>* __restrict__ velH,
-                const int32_t* __restrict__ idH, const uint32_t* __restrict__ offH,
-                const T* __restrict__ lx, const T* __restrict__ lv, T* __restrict__ lx_o,
-                T* __restrict__ lv_o, T* __restrict__ gk, T* __restrict__ ge) {
+    k_adj_step(int32_t ntab, const BlockInfo* __restrict__ btab,
+               const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ posS,
+               const Vec4<T>* __restrict__ velS, const int32_t* __restrict__ idS,
+               const uint32_t* __restrict__ offS, const int32_t* __restrict__ cellS,
+               const Vec4<T>* __restrict__ xpreS, const Vec4<T>* __restrict__ posH,
+               const Vec4<T>* __restrict__ velH, const int32_t* __restrict__ idH,
+               const uint32_t* __restrict__ offH, const TriGeo<T>* __restrict__ tgeo,
+               const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
+               const T* __restrict__ lx, const T* __restrict__ lv, T* __restrict__ lx_o,
+               T* __restrict__ lv_o, T* __restrict__ gk, T* __restrict__ ge,
+               T* __restrict__ gg) {
+  static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "lanes per particle must divide 64");
   const int32_t tb = xcd_tile_a(blockIdx.x, ntab);
   if (tb >= ntab) return;
   const BlockInfo bi = btab[tb];
-  const int32_t p = bi.p_start + (int32_t)threadIdx.x;
-  if (p >= bi.p_end) return;
+  const int32_t g = (int32_t)threadIdx.x % G;
+  const int32_t p = bi.p_start + (int32_t)threadIdx.x / G;
+  if (p >= bi.p_end) return;  // whole groups leave together
   const SceneDev<T>* __restrict__ s = scenes + bi.scene;
-  const GridDev& g = s->grid;
+  const GridDev& grid = s->grid;
   const Vec4<T> xi = posS[p], vi = velS[p];
   const int64_t me = idS[p];
   const T li[3] = {lv[3 * me], lv[3 * me + 1], lv[3 * me + 2]};
-  T ax[3] = {lx[3 * me], lx[3 * me + 1], lx[3 * me + 2]};
-  T av[3] = {li[0], li[1], li[2]};
+  T ax[3] = {T(0), T(0), T(0)}, av[3] = {T(0), T(0), T(0)};
   T pk = T(0), pe = T(0);
   uint32_t lo[9], hi[9];
   // (A) i's own kicks: members of the cells around home(i), bucket-sorted
   {
-    const int32_t hx = axis_index((double)xi.x, g.min[0], g.h, g.n[0]);
-    const int32_t hy = axis_index((double)xi.y, g.min[1], g.h, g.n[1]);
-    const int32_t hz = axis_index((double)xi.z, g.min[2], g.h, g.n[2]);
-    column_slices(g, hx, hy, hz, offS, lo, hi);
+    const int32_t hx = axis_index((double)xi.x, grid.min[0], grid.h, grid.n[0]);
+    const int32_t hy = axis_index((double)xi.y, grid.min[1], grid.h, grid.n[1]);
+    const int32_t hz = axis_index((double)xi.z, grid.min[2], grid.h, grid.n[2]);
+    column_slices(grid, hx, hy, hz, offS, lo, hi);
     for (int d = 0; d < 9; ++d)
-      for (uint32_t q = lo[d]; q < hi[d]; ++q) {
+      for (uint32_t q = lo[d] + (uint32_t)g; q < hi[d]; q += G) {
         if ((int32_t)q == p) continue;
         const Vec4<T> xj = posS[q];
         if (!touching(xi, xj)) continue;
@@ -135,12 +146,12 @@ __global__ void __launch_bounds__(256)
   }
   // (B) kicks i contributes to: particles whose home cell is around bucket(i), home-sorted
   {
-    const int32_t lc = cellS[p] - g.cell_base;
-    const int32_t bz = lc % g.n[2], rest = lc / g.n[2];
-    const int32_t by = rest % g.n[1], bx = rest / g.n[1];
-    column_slices(g, bx, by, bz, offH, lo, hi);
+    const int32_t lc = cellS[p] - grid.cell_base;
+    const int32_t bz = lc % grid.n[2], rest = lc / grid.n[2];
+    const int32_t by = rest % grid.n[1], bx = rest / grid.n[1];
+    column_slices(grid, bx, by, bz, offH, lo, hi);
     for (int d = 0; d < 9; ++d)
-      for (uint32_t q = lo[d]; q < hi[d]; ++q) {
+      for (uint32_t q = lo[d] + (uint32_t)g; q < hi[d]; q += G) {
         const int64_t j = idH[q];
         if (j == me) continue;
         const Vec4<T> xj = posH[q];
@@ -158,43 +169,27 @@ __global__ void __launch_bounds__(256)
         }
       }
   }
+  // fixed-order combination of the group's partial sums (identical on every lane)
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ax[c] += __shfl_xor(ax[c], off, 64);
+      av[c] += __shfl_xor(av[c], off, 64);
+    }
+    pk += __shfl_xor(pk, off, 64);
+    pe += __shfl_xor(pe, off, 64);
+  }
+  if (g != 0) return;
+  // adjoint of (x'', v''): the post-step cotangent plus the pair terms
+  T lxs[3], lvs[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    lx_o[3 * me + c] = ax[c];
-    lv_o[3 * me + c] = av[c];
+    lxs[c] = lx[3 * me + c] + ax[c];
+    lvs[c] = li[c] + av[c];
   }
-  gk[me] += pk;
-  ge[me] += pe;
-}
-
-// Container and integrate adjoint. Thread per particle of the pre-step layout (pos, vel, id =
-// state before the step). lx/lv: adjoint of (x'', v'') (original order); writes the adjoint of
-// (x, v) to lx_o/lv_o and adds dt * adjoint(v'_z) to gg (gravity z partials).
-template <class T>
-__global__ void __launch_bounds__(256)
-    k_adj_container(int32_t ntab, const BlockInfo* __restrict__ btab,
-                    const SceneDev<T>* __restrict__ scenes, const Vec4<T>* __restrict__ pos,
-                    const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
-                    const TriGeo<T>* __restrict__ tgeo, const TriBox<T>* __restrict__ tbox,
-                    const int32_t* __restrict__ tcell_off, const T* __restrict__ lx,
-                    const T* __restrict__ lv, T* __restrict__ lx_o, T* __restrict__ lv_o,
-                    T* __restrict__ gg) {
-  const int32_t tb = xcd_tile_a(blockIdx.x, ntab);
-  if (tb >= ntab) return;
-  const BlockInfo bi = btab[tb];
-  const int32_t p = bi.p_start + (int32_t)threadIdx.x;
-  if (p >= bi.p_end) return;
-  const SceneDev<T>* __restrict__ s = scenes + bi.scene;
-  const int64_t me = id[p];
-  Vec4<T> x = pos[p];
-  const Vec4<T> v = vel[p];
-  const T dt = s->dt;
-  // x' = x + v dt, v' = v + g dt (the forward's arithmetic, so the hit sets are the forward's)
-  x.x = x.x + v.x * dt;
-  x.y = x.y + v.y * dt;
-  x.z = x.z + v.z * dt;
-  T lxs[3] = {lx[3 * me], lx[3 * me + 1], lx[3 * me + 2]};
-  T lvs[3] = {lv[3 * me], lv[3 * me + 1], lv[3 * me + 2]};
+  // container adjoint at x' (the forward's arithmetic, so the hit sets are the forward's)
+  const Vec4<T> x = xpreS[p];
   if (s->mesh < 0) {
     const T r = x.w;
     const T px[3] = {x.x, x.y, x.z};
@@ -207,11 +202,10 @@ __global__ void __launch_bounds__(256)
       if (hmax) lvs[a] = -lvs[a] * s->emax[a];
     }
   } else {
-    const GridDev& g = s->grid;
-    const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-    const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-    const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-    const int32_t tc = s->tcell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+    const int32_t ix = axis_index((double)x.x, grid.min[0], grid.h, grid.n[0]);
+    const int32_t iy = axis_index((double)x.y, grid.min[1], grid.h, grid.n[1]);
+    const int32_t iz = axis_index((double)x.z, grid.min[2], grid.h, grid.n[2]);
+    const int32_t tc = s->tcell_base + (ix * grid.n[1] + iy) * grid.n[2] + iz;
     const int32_t k0 = tcell_off[tc], k1 = tcell_off[tc + 1];
     const Vec3<T> P = {x.x, x.y, x.z};
     const T r = x.w;
@@ -249,6 +243,9 @@ __global__ void __launch_bounds__(256)
     }
   }
   // integrate: lambda_x = lambda_x', lambda_v = lambda_v' + dt lambda_x'; d/dg_z += dt lambda_v'_z
+  const T dt = s->dt;
+  gk[me] += pk;
+  ge[me] += pe;
   gg[me] += dt * lvs[2];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -292,25 +289,35 @@ __global__ void __launch_bounds__(256) k_fill(int64_t n, T* __restrict__ a, T va
 static inline int32_t xgrid(int32_t tiles) { return ((tiles + 7) / 8) * 8; }
 
 template <class T>
-hipError_t launch_adj_pairs(hipStream_t st, int ntab, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const Vec4<T>* posS, const Vec4<T>* velS,
-                            const int32_t* idS, const uint32_t* offS, const int32_t* cellS,
-                            const Vec4<T>* posH, const Vec4<T>* velH, const int32_t* idH,
-                            const uint32_t* offH, const T* lx, const T* lv, T* lx_o, T* lv_o,
-                            T* gk, T* ge) {
-  hipLaunchKernelGGL(k_adj_pairs<T>, dim3(xgrid(ntab)), dim3(256), 0, st, ntab, btab, scenes, posS,
-                     velS, idS, offS, cellS, posH, velH, idH, offH, lx, lv, lx_o, lv_o, gk, ge);
+hipError_t launch_adj_step(hipStream_t st, int ntab, const BlockInfo* btab,
+                           const SceneDev<T>* scenes, const Vec4<T>* posS, const Vec4<T>* velS,
+                           const int32_t* idS, const uint32_t* offS, const int32_t* cellS,
+                           const Vec4<T>* xpreS, const Vec4<T>* posH, const Vec4<T>* velH,
+                           const int32_t* idH, const uint32_t* offH, const TriGeo<T>* tgeo,
+                           const TriBox<T>* tbox, const int32_t* tcell_off, const T* lx,
+                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg) {
+  hipLaunchKernelGGL((k_adj_step<T, kAdjLanes>), dim3(xgrid(ntab)), dim3(256), 0, st, ntab, btab,
+                     scenes, posS, velS, idS, offS, cellS, xpreS, posH, velH, idH, offH, tgeo,
+                     tbox, tcell_off, lx, lv, lx_o, lv_o, gk, ge, gg);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_adj_container(hipStream_t st, int ntab, const BlockInfo* btab,
-                                const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                                const int32_t* id, const TriGeo<T>* tgeo, const TriBox<T>* tbox,
-                                const int32_t* tcell_off, const T* lx, const T* lv, T* lx_o,
-                                T* lv_o, T* gg) {
-  hipLaunchKernelGGL(k_adj_container<T>, dim3(xgrid(ntab)), dim3(256), 0, st, ntab, btab, scenes,
-                     pos, vel, id, tgeo, tbox, tcell_off, lx, lv, lx_o, lv_o, gg);
+__global__ void __launch_bounds__(256)
+    k_copy2(int64_t n, const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ a_o,
+            T* __restrict__ b_o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a_o[i] = a[i];
+    b_o[i] = b[i];
+  }
+}
+
+template <class T>
+hipError_t launch_copy2(hipStream_t st, int64_t n, const T* a, const T* b, T* a_o, T* b_o) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy2<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b,
+                     a_o, b_o);
   return hipGetLastError();
 }
 
@@ -329,15 +336,13 @@ hipError_t launch_fill(hipStream_t st, int64_t n, T* a, T value) {
 }
 
 #define PSX_ADJ_INSTANTIATE(T)                                                                   \
-  template hipError_t launch_adj_pairs<T>(hipStream_t, int, const BlockInfo*, const SceneDev<T>*, \
-                                          const Vec4<T>*, const Vec4<T>*, const int32_t*,        \
-                                          const uint32_t*, const int32_t*, const Vec4<T>*,       \
-                                          const Vec4<T>*, const int32_t*, const uint32_t*,       \
-                                          const T*, const T*, T*, T*, T*, T*);                   \
-  template hipError_t launch_adj_container<T>(hipStream_t, int, const BlockInfo*,               \
-                                              const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,\
-                                              const int32_t*, const TriGeo<T>*, const TriBox<T>*,\
-                                              const int32_t*, const T*, const T*, T*, T*, T*);   \
+  template hipError_t launch_adj_step<T>(hipStream_t, int, const BlockInfo*, const SceneDev<T>*, \
+                                         const Vec4<T>*, const Vec4<T>*, const int32_t*,         \
+                                         const uint32_t*, const int32_t*, const Vec4<T>*,        \
+                                         const Vec4<T>*, const Vec4<T>*, const int32_t*,         \
+                                         const uint32_t*, const TriGeo<T>*, const TriBox<T>*,    \
+                                         const int32_t*, const T*, const T*, T*, T*, T*, T*, T*); \
+  template hipError_t launch_copy2<T>(hipStream_t, int64_t, const T*, const T*, T*, T*);         \
   template hipError_t launch_reduce3<T>(hipStream_t, int32_t, const T*, const T*, const T*,      \
                                         double*);                                                \
   template hipError_t launch_fill<T>(hipStream_t, int64_t, T*, T);

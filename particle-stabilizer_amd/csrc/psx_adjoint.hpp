@@ -5,20 +5,25 @@
 
 namespace psx {
 
-template <class T>
-hipError_t launch_adj_pairs(hipStream_t st, int ntab, const BlockInfo* btab,
-                            const SceneDev<T>* scenes, const Vec4<T>* posS, const Vec4<T>* velS,
-                            const int32_t* idS, const uint32_t* offS, const int32_t* cellS,
-                            const Vec4<T>* posH, const Vec4<T>* velH, const int32_t* idH,
-                            const uint32_t* offH, const T* lx, const T* lv, T* lx_o, T* lv_o,
-                            T* gk, T* ge);
+// Lanes per particle of the adjoint step kernel (its workgroups hold 256 / kAdjLanes particles).
+constexpr int kAdjLanes = 8;
+constexpr int kAdjPerBlock = 256 / kAdjLanes;
 
+// Adjoint of one step (pair, container and integrate terms) for the bucket-sorted snapshot of the
+// step: btab holds kAdjPerBlock particles per workgroup. xpreS: integrated, pre-container positions
+// of the snapshot's slots.
 template <class T>
-hipError_t launch_adj_container(hipStream_t st, int ntab, const BlockInfo* btab,
-                                const SceneDev<T>* scenes, const Vec4<T>* pos, const Vec4<T>* vel,
-                                const int32_t* id, const TriGeo<T>* tgeo, const TriBox<T>* tbox,
-                                const int32_t* tcell_off, const T* lx, const T* lv, T* lx_o,
-                                T* lv_o, T* gg);
+hipError_t launch_adj_step(hipStream_t st, int ntab, const BlockInfo* btab,
+                           const SceneDev<T>* scenes, const Vec4<T>* posS, const Vec4<T>* velS,
+                           const int32_t* idS, const uint32_t* offS, const int32_t* cellS,
+                           const Vec4<T>* xpreS, const Vec4<T>* posH, const Vec4<T>* velH,
+                           const int32_t* idH, const uint32_t* offH, const TriGeo<T>* tgeo,
+                           const TriBox<T>* tbox, const int32_t* tcell_off, const T* lx,
+                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg);
+
+// a_o[i] = a[i], b_o[i] = b[i] for i < n.
+template <class T>
+hipError_t launch_copy2(hipStream_t st, int64_t n, const T* a, const T* b, T* a_o, T* b_o);
 
 template <class T>
 hipError_t launch_reduce3(hipStream_t st, int32_t n, const T* a, const T* b, const T* c,

@@ -510,7 +510,8 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
                   const int32_t* __restrict__ tcell_off, uint32_t* __restrict__ flags, int mode,
                   const uint8_t* __restrict__ cull, int32_t n, Vec4<T>* __restrict__ pos_o,
                   Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
-                  int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o) {
+                  int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o,
+                  Vec4<T>* __restrict__ xpre_o) {
   static_assert(kCellIds == 8, "the id block is read as two int4");
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -562,7 +563,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     }
     // The container acts on the pre-container state the bucketing hashed; its triangle-list loads
     // run alongside the rank's offset and id-block loads.
-    const T x0 = x.x, y0 = x.y, z0 = x.z;
+    const Vec4<T> xpre = x;
     if (mesh) {
       mesh_contact_wave(&s, active, k0, k1, tent, x, v);
     } else if (mode & kModeContainer) {
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     // home cell of the post-container position for the pair kernel (the bucket cell unless the
     // container moved the particle), with the cell's cull flag in bit 31
     int32_t home = c;
-    if (!(x.x == x0 && x.y == y0 && x.z == z0)) {
+    if (!(x.x == xpre.x && x.y == xpre.y && x.z == xpre.z)) {
       const GridDev& g = s.grid;
       const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
       const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
@@ -581,6 +582,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       cull_c = cull[home];
     }
     home_o[dst] = (uint32_t)home | (cull_c ? kHomeCull : 0u);
+    if (xpre_o) xpre_o[dst] = xpre;
     if (mode & kModeCheck) {
       const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
                       isfinite(v.y) && isfinite(v.z);
@@ -994,13 +996,13 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o, uint32_t* home_o) {
+                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
              cell_ids, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode, cull, n,  \
-             pos_o, vel_o, id_o, cell_o, home_o)
+             pos_o, vel_o, id_o, cell_o, home_o, xpre_o)
   if (one) {
     if (blocks) PSX_RANK(false, true);
     else PSX_RANK(false, false);
@@ -1079,7 +1081,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                             const int32_t*, const TriEntry<T>*,                \
                                             const int32_t*, uint32_t*, int, const uint8_t*,    \
                                             Vec4<T>*, Vec4<T>*, int32_t*, int32_t*,            \
-                                            uint32_t*);                                        \
+                                            uint32_t*, Vec4<T>*);                              \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \

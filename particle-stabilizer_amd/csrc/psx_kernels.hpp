@@ -63,7 +63,8 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 // cell-sorted order, with the bucket cell (cell_o, optional) and the home word of the gathered
 // position (home_o, for the pair kernel: home cell | kHomeCull when cull[home cell] is set) of
 // every sorted slot. cull: one byte per cell, nonzero where the pair kernel may cull the
-// neighbour cells by distance.
+// neighbour cells by distance. xpre_o (optional): the integrated, pre-container position of every
+// sorted slot (recorded for the adjoint).
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
@@ -73,7 +74,7 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o, uint32_t* home_o);
+                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o);
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
 // home: the home word of every sorted slot (from the rank/gather pass that produced pos).

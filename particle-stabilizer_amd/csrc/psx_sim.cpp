@@ -532,7 +532,8 @@ struct SimImpl final : SimBase {
   }
 
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
-              int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof) {
+              int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
+              Vec4<T>* xpre_sorted = nullptr) {
     int nb_i, nb_s;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
@@ -562,7 +563,7 @@ struct SimImpl final : SimBase {
                                    ovf.as<int2>(), ovf_ctr.as<uint32_t>(), list.as<int32_t>(), ps,
                                    vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
-                                   cell_sorted, home.as<uint32_t>());
+                                   cell_sorted, home.as<uint32_t>(), xpre_sorted);
     });
   }
 
@@ -610,7 +611,7 @@ struct SimImpl final : SimBase {
   // recompute copies nothing. Both halves are replayed from hipGraphs captured per segment length.
   int64_t win_steps = 0, win_every = 0, win_nseg = 0;
   DevBuf ck_pos, ck_vel, ck_id, seg_pos, seg_vel, seg_id;
-  DevBuf rs_pos, rs_vel, rs_id, rs_cell, rs_off, rs_vnew;
+  DevBuf rs_pos, rs_vel, rs_id, rs_cell, rs_off, rs_vnew, rs_xpre;
   DevBuf posH, velH, idH, offH;
   DevBuf lxA, lvA, lxB, lvB, gk, ge, gg, gout;
   std::map<int64_t, hipGraphExec_t> g_replay, g_adjoint;
@@ -653,6 +654,7 @@ struct SimImpl final : SimBase {
       rs_pos.alloc(((size_t)every * nn + kPairPad) * sizeof(Vec4<T>));
       rs_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_vnew.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_xpre.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_id.alloc((size_t)every * nn * 4);
       rs_cell.alloc((size_t)every * nn * 4);
       rs_off.alloc((size_t)every * ((size_t)cpad + 1) * 4);  // k_scan writes cpad + 1
@@ -689,15 +691,19 @@ struct SimImpl final : SimBase {
       uint32_t* so = rs_off.as<uint32_t>() + k * nc;
       bucket(pre_pos(k), pre_vel(k), pre_id(k), sp, sv,
              rs_id.as<int32_t>() + k * nn, so, rs_cell.as<int32_t>() + k * nn,
-             kModeIntegrate | kModeContainer | kModeCheck, nullptr);
+             kModeIntegrate | kModeContainer | kModeCheck, nullptr,
+             rs_xpre.as<Vec4<T>>() + k * nn);
       pairs(sp, sv, so, rs_vnew.as<Vec4<T>>() + k * nn, nullptr);
     }
   }
 
+  // Adjoint sweep of a recorded segment, last step first; the cotangent alternates between the A
+  // and B buffers and ends in A.
   void adjoint_segment(int64_t ks) {
     const size_t nn = (size_t)n, nc = (size_t)cpad + 1;  // offset record stride
     int nb;
-    const BlockInfo* bt = table(256, &nb);
+    const BlockInfo* bt = table(kAdjPerBlock, &nb);
+    T *lx = lxA.as<T>(), *lv = lvA.as<T>(), *lx_o = lxB.as<T>(), *lv_o = lvB.as<T>();
     for (int64_t k = ks - 1; k >= 0; --k) {
       Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
@@ -705,16 +711,18 @@ struct SimImpl final : SimBase {
       // home-cell bucketing of the snapshot (hash only: nothing is written to sp/sv)
       bucket(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
              offH.as<uint32_t>(), nullptr, 0, nullptr);
-      PSX_HIP(launch_adj_pairs<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
-                                  rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,
-                                  posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
-                                  offH.as<uint32_t>(), lxA.as<T>(), lvA.as<T>(), lxB.as<T>(),
-                                  lvB.as<T>(), gk.as<T>(), ge.as<T>()));
-      PSX_HIP(launch_adj_container<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), pre_pos(k),
-                                      pre_vel(k), pre_id(k), tgeo_d.as<TriGeo<T>>(),
-                                      tbox_d.as<TriBox<T>>(), tcell_off_d.as<int32_t>(),
-                                      lxB.as<T>(), lvB.as<T>(), lxA.as<T>(), lvA.as<T>(), gg.as<T>()));
+      PSX_HIP(launch_adj_step<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
+                                 rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,
+                                 rs_xpre.as<Vec4<T>>() + k * nn, posH.as<Vec4<T>>(),
+                                 velH.as<Vec4<T>>(), idH.as<int32_t>(), offH.as<uint32_t>(),
+                                 tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
+                                 tcell_off_d.as<int32_t>(), lx, lv, lx_o, lv_o, gk.as<T>(),
+                                 ge.as<T>(), gg.as<T>()));
+      std::swap(lx, lx_o);
+      std::swap(lv, lv_o);
     }
+    if (lx != lxA.as<T>())
+      PSX_HIP(launch_copy2<T>(stream, 3 * (int64_t)nn, lx, lv, lxA.as<T>(), lvA.as<T>()));
   }
 
   template <class F>
@@ -754,7 +762,7 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpyAsync(lvA.p, dlv, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
     for (DevBuf* b : {&gk, &ge, &gg}) PSX_HIP(launch_fill<T>(stream, (int64_t)nn, b->as<T>(), T(0)));
     int nb;
-    table(256, &nb);
+    table(kAdjPerBlock, &nb);  // created here, outside any graph capture
     std::vector<hipEvent_t> ev((size_t)(3 * win_nseg), nullptr);
     for (auto& e : ev) PSX_HIP(hipEventCreate(&e));
     struct EvGuard {
