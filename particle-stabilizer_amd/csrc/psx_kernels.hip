@@ -554,8 +554,11 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
             rank += (q.x == c && q.y < me) ? 1u : 0u;
           }
         }
-      } else {
-        for (uint32_t k = b; k < e; ++k) rank += (list[k] < me) ? 1u : 0u;
+      } else {  // the first four members in one go (the list carries kListPad spare entries)
+        const int32_t l0 = list[b], l1 = list[b + 1], l2 = list[b + 2], l3 = list[b + 3];
+        rank = ((b < e && l0 < me) ? 1u : 0u) + ((b + 1 < e && l1 < me) ? 1u : 0u) +
+               ((b + 2 < e && l2 < me) ? 1u : 0u) + ((b + 3 < e && l3 < me) ? 1u : 0u);
+        for (uint32_t k = b + 4; k < e; ++k) rank += (list[k] < me) ? 1u : 0u;
       }
       // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
       // arrays even if a stalled scan (reported through the flags) left them short

@@ -25,6 +25,9 @@ constexpr int kMaxBlockSort = sizeof(T) == 4 ? 1024 : 512;
 // at a time).
 constexpr int kTriPad = 4;
 
+// Spare entries after the dense cell lists (the rank kernel reads a cell's first four at once).
+constexpr int kListPad = 4;
+
 // Spare entries after every sorted position array the pair kernel reads (its chunked loads may
 // reach three entries past the last particle).
 constexpr int kPairPad = 4;

@@ -460,7 +460,7 @@ struct SimImpl final : SimBase {
     cell_of.alloc((size_t)nn * 4);
     home.alloc((size_t)nn * 4);
     slot_of.alloc((size_t)nn * 4);
-    list.alloc((size_t)nn * 4);
+    list.alloc(((size_t)nn + kListPad) * 4);
     cell_ids.alloc((size_t)cpad * kCellIds * 4);
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
