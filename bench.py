@@ -25,6 +25,15 @@ sys.path.insert(0, str(REPO))
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+# launch shapes chosen by psx.tune.autotune on MI355X for these workloads (applied without timing;
+# --retune sweeps again)
+TUNE_CACHE = REPO / "particle-stabilizer_amd" / "psx" / "tuning_mi355x.json"
+RETUNE = False
+
+
+def tuned(sim) -> dict:
+    from psx.tune import autotune
+    return autotune(sim, cache=TUNE_CACHE, force=RETUNE)["tuning"]
 METRIC = "particle-steps/s on 1 GPU (plain vs batched); adjoint/forward step time ratio"
 
 
@@ -79,8 +88,7 @@ def bench_hopper(steps: int = 100, warmup: int = 10) -> dict:
     sc = scenes.hopper_settle()
     n = sc.radius.shape[0]
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
-    from psx.tune import autotune
-    tuning = autotune(sim)["tuning"]
+    tuning = tuned(sim)
     sim.step(warmup)
     p0, v0 = sim.state()
     dt = _timed(sim, steps)
@@ -106,8 +114,7 @@ def bench_batched(steps: int = 100, warmup: int = 10) -> dict:
     scs = scenes.batched_boxes()
     total = sum(sc.radius.shape[0] for sc in scs)
     b = psx.Batch(scs)
-    from psx.tune import autotune
-    tuning = autotune(b)["tuning"]
+    tuning = tuned(b)
     b.step(warmup)
     tb = _timed(b, steps)
     cells = b.num_cells
@@ -165,8 +172,7 @@ def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
     sc = scenes.calibration_scene()
     n = sc.radius.shape[0]
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
-    from psx.tune import autotune
-    autotune(sim)
+    tuned(sim)
     dev = torch.device("cuda")
     gx = torch.randn((n, 3), device=dev, dtype=torch.float32)
     gv = torch.randn((n, 3), device=dev, dtype=torch.float32)
@@ -209,7 +215,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default launch shapes instead of autotuning them")
+                    help="keep the default launch shapes instead of the tuned ones")
+    ap.add_argument("--retune", action="store_true",
+                    help="sweep the launch shapes again instead of applying the tuning file")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hopper, batched and adjoint measurements")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -234,9 +242,11 @@ def main() -> int:
 
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
     sim.set_tuning(use_graph=0 if args.no_graph else 1)
+    global RETUNE
+    RETUNE = args.retune
     if not args.no_tune:
-        from psx.tune import autotune
-        autotune(sim, use_graph=not args.no_graph)
+        tuned(sim)
+        sim.set_tuning(use_graph=0 if args.no_graph else 1)
     tuning = sim.tuning
     sim.step(args.warmup)
     sim.synchronize()

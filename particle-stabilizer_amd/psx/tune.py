@@ -117,3 +117,34 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
         path.parent.mkdir(parents=True, exist_ok=True)
         path.write_text(json.dumps(data, indent=1, sort_keys=True))
     return {"tuning": tuning, "kernel_ms": kernel_ms, "cached": False}
+
+
+def _main() -> int:
+    """python -m psx.tune --out FILE [--workloads ...]: tunes the standard workloads of
+    psx.scenes on this GPU and stores their launch shapes in FILE."""
+    import argparse
+    import sys
+
+    from . import scenes
+    from .sim import Batch, Simulation
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--workloads", nargs="*",
+                    default=["cylinder_settle", "hopper_settle", "batched_boxes",
+                             "calibration_scene"])
+    args = ap.parse_args()
+    for name in args.workloads:
+        made = getattr(scenes, name)()
+        if isinstance(made, list):
+            sim = Batch(made)
+        else:
+            sim = Simulation(made.pos, made.vel, made.radius, made.mass, made.params, made.mesh)
+        out = autotune(sim, cache=args.out, force=True)
+        print(f"{name}: {signature(sim)} -> {out['tuning']}", file=sys.stderr, flush=True)
+        sim.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
