@@ -148,6 +148,23 @@ int psx_get_grid(const psx_sim* sim, int32_t scene, double* min3, double* h, int
 int psx_debug_bucketing(psx_sim* sim, int32_t* cell, uint32_t* count, uint32_t* offset,
                         int32_t* perm);
 
+/* ---- statistics (ParticleStatistics::calculateStatistics, ParticleStatistics.cpp:3-43) --------- */
+
+typedef struct psx_stats {
+  double kinetic_energy;  /* sum of 0.5 m (v . v) */
+  double min[3];          /* AABB of the spheres: min of (x - r), max of (x + r) per axis */
+  double max[3];
+  int64_t n_collided;     /* particles i with a partner j > i (original index) such that
+                             r_i + r_j - |x_j - x_i| > 0 */
+  double max_overlap;     /* the largest such r_i + r_j - |x_j - x_i|; 0 when there is none */
+} psx_stats;
+
+/* Statistics of the current state of every scene into out[0 .. num_scenes). The pair part runs on a
+ * bucketing of the current positions (O(N), the reference's loop is O(N^2)); everything is evaluated
+ * in binary64 and combined in a fixed order (the kinetic-energy sum is therefore deterministic but
+ * not the reference's sequential rounding). Waits for the stream. */
+int psx_statistics(psx_sim* sim, psx_stats* out);
+
 /* ---- differentiable windows ---------------------------------------------------------------- */
 /* No counterpart in the reference. Gradients of a loss on the state after a window of steps with
  * respect to the initial state and to spring, sphere restitution and gravity_z, holding each step's
@@ -201,6 +218,20 @@ int psx_write_csv(const char* path, int64_t n, const double* pos, const double* 
 int psx_read_obj(const char* path, int64_t* n_triangles, double** vertices);
 
 void psx_free(void* p);
+
+/* ---- post-run outputs (host, after a simulation; ParticleStabilizerMain.cpp:275-296) ----------- */
+
+/* ParticleModel::resolveOverlaps (ParticleModel.cpp:154-252): lifts overlapping particles along
+ * `axis` (0, 1, 2) in the order of their coordinate on that axis; pos (n x 3) is updated in place.
+ * Afterwards *n_collided / *max_overlap (either may be NULL) receive the overlap check the reference
+ * logs: particles with a partner of higher index closer than r_i + r_j, and the largest overlap. */
+int psx_resolve_overlaps(int64_t n, double* pos, const double* radius, int32_t axis,
+                         int64_t* n_collided, double* max_overlap);
+
+/* ObjData::write (ObjData.cpp:8-140): every particle as a triangulated UV sphere in a Wavefront OBJ
+ * file (shared vertex normals; resolution ndivs = max((int)sqrt(1e6 / (6 + n)) + 1, 6)); parent
+ * directories are created. */
+int psx_write_obj_spheres(const char* path, int64_t n, const double* pos, const double* radius);
 
 /* ---- errors / version ---------------------------------------------------------------------- */
 

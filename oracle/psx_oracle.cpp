@@ -555,6 +555,119 @@ void oracle_mesh_normals_f64(int64_t ntri, const double* tri, double* normals) {
   }
 }
 
+// ParticleStatistics::calculateStatistics (ParticleStatistics.cpp:3-43) over n particles in index
+// order: out = {kinetic energy, min x, y, z, max x, y, z, collided count, max overlap}. The
+// kinetic energy is the sequential sum of 0.5 * m * dot(v, v); the AABB uses glm::min / glm::max
+// ((b < a) ? b : a and (a < b) ? b : a) starting from particle 0; particle i counts as collided
+// when some j > i has r_i + r_j - distance(x_i, x_j) > 0, distance = length(x_j - x_i).
+void oracle_statistics(int64_t n, const double* pos, const double* vel, const double* radius,
+                       const double* mass, double* out) {
+  double ke = 0.0;
+  double mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = pos[a] - radius[0];
+    mx[a] = pos[a] + radius[0];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<double> v{vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    ke += 0.5 * mass[i] * dot(v, v);
+    for (int a = 0; a < 3; ++a) {
+      const double lo = pos[3 * i + a] - radius[i], hi = pos[3 * i + a] + radius[i];
+      mn[a] = (lo < mn[a]) ? lo : mn[a];
+      mx[a] = (mx[a] < hi) ? hi : mx[a];
+    }
+  }
+  int64_t collided = 0;
+  double maxov = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<double> xi{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    bool hit = false;
+    for (int64_t j = i + 1; j < n; ++j) {
+      const V3<double> xj{pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+      const double dist = length(xj - xi);
+      const double cd = radius[i] + radius[j] - dist;
+      if (cd > 0.0) {
+        hit = true;
+        maxov = std::max(maxov, cd);
+      }
+    }
+    if (hit) ++collided;
+  }
+  out[0] = ke;
+  for (int a = 0; a < 3; ++a) {
+    out[1 + a] = mn[a];
+    out[4 + a] = mx[a];
+  }
+  out[7] = (double)collided;
+  out[8] = maxov;
+}
+
+// ParticleModel::resolveOverlaps (ParticleModel.cpp:154-252), literally: rank by the coordinate
+// along `axis` with std::sort; for every rank i, the partner scan starts at the first rank j < i
+// whose top (coordinate + radius) reaches the bottom of particle i (rank 0 when none does); each
+// overlapping partner lifts particle i to x_j + sqrt((R^2 - d0^2) - d1^2) along `axis`; a lift
+// whose new coordinate falls strictly between the coordinates at ranks k-1 and k (first such
+// k > i) re-sorts ranks [i, k] and processes rank i again. Then the O(N^2) overlap check:
+// out = {collided count, max overlap}.
+void oracle_resolve_overlaps(int64_t n, double* pos, const double* radius, int32_t axis,
+                             double* out) {
+  std::vector<int64_t> rank((size_t)n);
+  for (int64_t i = 0; i < n; ++i) rank[(size_t)i] = i;
+  auto less = [&](int64_t a, int64_t b) { return pos[3 * a + axis] < pos[3 * b + axis]; };
+  std::sort(rank.begin(), rank.end(), less);
+  int other[2], k = 0;
+  for (int a = 0; a < 3; ++a)
+    if (a != axis) other[k++] = a;
+  for (int64_t i = 0; i < n; ++i) {
+    double* pi = pos + 3 * rank[(size_t)i];
+    const double ri = radius[rank[(size_t)i]];
+    int64_t first = 0;
+    for (int64_t j = 0; j < i; ++j) {
+      const int64_t q = rank[(size_t)j];
+      if (pi[axis] - ri <= pos[3 * q + axis] + radius[q]) {
+        first = j;
+        break;
+      }
+    }
+    for (int64_t j = first; j < i; ++j) {
+      const double* pj = pos + 3 * rank[(size_t)j];
+      const double sum = ri + radius[rank[(size_t)j]];
+      const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+      if (!(sum * sum - ((dx * dx + dy * dy) + dz * dz) > 0.0)) continue;
+      const double e0 = pi[other[0]] - pj[other[0]], e1 = pi[other[1]] - pj[other[1]];
+      pi[axis] = pj[axis] + std::sqrt((sum * sum - e0 * e0) - e1 * e1);
+      int64_t last = i;
+      for (int64_t m = i + 1; m < n; ++m)
+        if (pos[3 * rank[(size_t)(m - 1)] + axis] < pi[axis] && pi[axis] < pos[3 * rank[(size_t)m] + axis]) {
+          last = m;
+          break;
+        }
+      if (last != i) {
+        std::sort(rank.begin() + i, rank.begin() + last + 1, less);
+        --i;
+        break;
+      }
+    }
+  }
+  int64_t collided = 0;
+  double worst = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    bool hit = false;
+    for (int64_t j = i + 1; j < n; ++j) {
+      const double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+      const double cd = radius[i] + radius[j] - std::sqrt((dx * dx + dy * dy) + dz * dz);
+      if (cd > 0.0) {
+        hit = true;
+        worst = std::max(worst, cd);
+      }
+    }
+    collided += hit;
+  }
+  out[0] = (double)collided;
+  out[1] = worst;
+}
+
 // Mass of ParticleData::readFromCSV (ParticleIO.cpp:131).
 double oracle_mass(double density, double radius) {
   return density * 4.0 / 3.0 * M_PI * std::pow(radius, 3);

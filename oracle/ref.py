@@ -6,6 +6,7 @@ line it restates.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 from pathlib import Path
@@ -143,3 +144,79 @@ def mass(density: float, radius: float) -> float:
 
 def max_threads() -> int:
     return int(load().oracle_max_threads())
+
+
+def statistics(pos, vel, radius, mass) -> dict:
+    """ParticleStatistics::calculateStatistics (ParticleStatistics.cpp:3-43), O(N^2)."""
+    lib = load()
+    lib.oracle_statistics.argtypes = [C.c_int64, _DP, _DP, _DP, _DP, _DP]
+    lib.oracle_statistics.restype = None
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+    radius = np.ascontiguousarray(radius, np.float64).reshape(-1)
+    mass = np.ascontiguousarray(mass, np.float64).reshape(-1)
+    out = np.zeros(9)
+    lib.oracle_statistics(pos.shape[0], _p(pos, C.c_double), _p(vel, C.c_double),
+                          _p(radius, C.c_double), _p(mass, C.c_double), _p(out, C.c_double))
+    return {"kinetic_energy": out[0], "min": out[1:4].copy(), "max": out[4:7].copy(),
+            "n_collided": int(out[7]), "max_overlap": out[8]}
+
+
+def resolve_overlaps(pos, radius, axis: int):
+    """ParticleModel::resolveOverlaps (ParticleModel.cpp:154-252), literal O(N^2) restatement:
+    returns (lifted positions, collided count, max overlap)."""
+    lib = load()
+    lib.oracle_resolve_overlaps.argtypes = [C.c_int64, _DP, _DP, C.c_int32, _DP]
+    lib.oracle_resolve_overlaps.restype = None
+    pos = np.array(pos, np.float64).reshape(-1, 3)
+    radius = np.ascontiguousarray(radius, np.float64).reshape(-1)
+    out = np.zeros(2)
+    lib.oracle_resolve_overlaps(pos.shape[0], _p(pos, C.c_double), _p(radius, C.c_double),
+                                int(axis), _p(out, C.c_double))
+    return pos, int(out[0]), float(out[1])
+
+
+def obj_sphere_template(n: int):
+    """ObjData::write (ObjData.cpp:20-73): the UV-sphere template for n particles, returned as
+    (ndivs, unit vertices (3 * faces, 3) float32). Angles and products in binary32, sin / cos by the
+    C library's sinf / cosf; theta + dtheta and phi + dphi are rounded sums as in the reference."""
+    libm = C.CDLL("libm.so.6")
+    for fn in ("sinf", "cosf"):
+        getattr(libm, fn).argtypes = [C.c_float]
+        getattr(libm, fn).restype = C.c_float
+    f32 = np.float32
+    ndivs = max(int(math.sqrt(1000000.0 / (6.0 + n))) + 1, 6)
+    nth, nph = ndivs, 2 * ndivs
+    dth = f32(math.pi / float(f32(nth - 1)))
+    dph = f32(2 * math.pi / float(f32(nph - 1)))
+    th = np.arange(nth - 1, dtype=f32) * dth
+    th1 = th + dth
+    ph = np.arange(nph - 1, dtype=f32) * dph
+    ph1 = ph + dph
+    trig = lambda fn, a: np.array([getattr(libm, fn)(float(x)) for x in a], f32)  # noqa: E731
+    sin_t, cos_t, sin_t1, cos_t1 = trig("sinf", th), trig("cosf", th), trig("sinf", th1), trig("cosf", th1)
+    sin_p, cos_p, sin_p1, cos_p1 = trig("sinf", ph), trig("cosf", ph), trig("sinf", ph1), trig("cosf", ph1)
+
+    def unit(st, ct, sp, cp):  # (nth-1, nph-1, 3)
+        a, b = np.meshgrid(np.arange(nth - 1), np.arange(nph - 1), indexing="ij")
+        return np.stack([st[a] * cp[b], st[a] * sp[b], ct[a]], axis=-1).astype(f32)
+
+    q0 = unit(sin_t, cos_t, sin_p, cos_p)
+    q1 = unit(sin_t1, cos_t1, sin_p, cos_p)
+    q2 = unit(sin_t1, cos_t1, sin_p1, cos_p1)
+    q3 = unit(sin_t, cos_t, sin_p1, cos_p1)
+    tpl = np.stack([q0, q1, q2, q2, q3, q0], axis=2)  # (nth-1, nph-1, 6, 3)
+    return ndivs, tpl.reshape(-1, 3)
+
+
+def obj_vertex(center, radius, q) -> tuple:
+    """One "v" entry of ObjData::write: radius * q + centre in binary32 (ObjData.cpp:84-100)."""
+    f32 = np.float32
+    c = [f32(x) for x in center]
+    r = f32(radius)
+    return tuple(float(f32(f32(r * f32(q[a])) + c[a])) for a in range(3))
+
+
+def stream_float(x: float) -> str:
+    """A float or double as an output stream with default flags prints it (%g, precision 6)."""
+    return "%g" % x

@@ -8,7 +8,9 @@
 // like -h; --coeff-of-restitution-wall is read in its documented order Z_MIN Z_MAX X_MIN X_MAX
 // Y_MIN Y_MAX (the reference's parenthesised constructor call leaves the order unspecified and
 // g++ reverses it); a flag missing its value is an error instead of reading past argv;
-// --dtype f32|f64 picks the device arithmetic (default f64, the reference's scalar type).
+// --dtype f32|f64 picks the device arithmetic (default f64, the reference's scalar type); an
+// --resolve-overlap-axis other than 0, 1, 2 (negative: off) is an error instead of an out-of-range
+// vector index.
 #include <dlfcn.h>
 #include <limits.h>
 #include <unistd.h>
@@ -249,22 +251,46 @@ int main(int argc, char** argv) {
       args.coefficientOfSpring, args.coefficientOfRestitutionSphere, args.dt,
       (size_t)args.maxSteps, args.toVisualize, result);
 
-  if (args.resolveOverlapAxis >= 0)
-    std::cerr << "[psx] --resolve-overlap-axis: the overlap post-process is not part of this build; skipped\n";
+  std::vector<double> p(3 * (size_t)n), r((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    p[3 * i] = result[(size_t)i].position.x;
+    p[3 * i + 1] = result[(size_t)i].position.y;
+    p[3 * i + 2] = result[(size_t)i].position.z;
+    r[(size_t)i] = result[(size_t)i].radius;
+  }
+
+  // ParticleStabilizerMain.cpp:275-296: overlap post-process, then the CSV, then the OBJ
+  if (args.resolveOverlapAxis >= 0) {
+    using resolve_t = int (*)(int64_t, double*, const double*, int32_t, int64_t*, double*);
+    auto resolve = reinterpret_cast<resolve_t>(dlsym(handle, "psx_resolve_overlaps"));
+    if (!resolve) die("library lacks psx_resolve_overlaps");
+    std::cout << "[psx] ### Resolving overlaps\n";
+    int64_t collided = 0;
+    double worst = 0.0;
+    if (resolve(n, p.data(), r.data(), args.resolveOverlapAxis, &collided, &worst) != 0)
+      die(last_error());
+    char line[128];
+    std::snprintf(line, sizeof line, "%.15f", worst);
+    std::cout << "[psx] Number of collided particles: " << collided << " / " << n << "\n"
+              << "[psx] Max collision distance: " << line << "\n"
+              << "[psx] ### Finished resolving overlaps\n";
+  }
 
   if (!args.particleOutPath.empty()) {
-    std::vector<double> p(3 * (size_t)n), r((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-      p[3 * i] = result[(size_t)i].position.x;
-      p[3 * i + 1] = result[(size_t)i].position.y;
-      p[3 * i + 2] = result[(size_t)i].position.z;
-      r[(size_t)i] = result[(size_t)i].radius;
-    }
     if (write_csv(args.particleOutPath.c_str(), n, p.data(), r.data()) != 0) die(last_error());
     std::cout << "[psx] Finished writing to file " << args.particleOutPath << "\n";
   }
-  if (!args.objOutPath.empty())
-    std::cerr << "[psx] --out-obj-path: sphere OBJ export is not part of this build; skipped\n";
+  if (!args.objOutPath.empty()) {
+    using write_obj_t = int (*)(const char*, int64_t, const double*, const double*);
+    auto write_obj = reinterpret_cast<write_obj_t>(dlsym(handle, "psx_write_obj_spheres"));
+    if (!write_obj) die("library lacks psx_write_obj_spheres");
+    std::cout << "[psx] Writing to file " << args.objOutPath << "\n";
+    // ObjData::write logs a file that cannot be opened and carries on
+    if (write_obj(args.objOutPath.c_str(), n, p.data(), r.data()) != 0)
+      std::cerr << "[psx] " << last_error() << "\n";
+    else
+      std::cout << "[psx] Finished writing to file " << args.objOutPath << "\n";
+  }
   std::cout << "[psx] Simulation completed successfully.\n";
   return EXIT_SUCCESS;
 }

@@ -1,6 +1,6 @@
 // psx_compat.cpp -- runSimulateHIP, the reference's plugin signature (psx_compat.hpp) implemented on
 // the C ABI. Mirrors runSimulateCUDA (PhysicsEngineCUDA.cpp:86-175): build the model and the
-// container, run maxSteps steps, copy the particles out by index.
+// container, run maxSteps steps with the periodic statistics log, copy the particles out by index.
 #include "psx_compat.hpp"
 
 #include <chrono>
@@ -107,9 +107,22 @@ extern "C" void runSimulateHIP(const std::vector<psxcompat::ParticlePtr>& partic
     done += k;
     const auto now = std::chrono::high_resolution_clock::now();
     if (std::chrono::duration<double>(now - t_prev).count() > 10.0 || done == maxSteps) {
-      if (psx_synchronize(sim) != PSX_OK) critical("psx_synchronize");
-      log_info("Step " + std::to_string(done) + ", Elapsed time: " +
-               std::to_string(std::chrono::duration<double>(now - t_start).count()) + " [sec]");
+      // ParticleStatistics::calculateStatistics (ParticleStatistics.cpp:3-43), computed on the
+      // GPU over the particle grid, logged as PhysicsEngine.cpp:59-64 does
+      psx_stats st;
+      if (psx_statistics(sim, &st) != PSX_OK) critical("psx_statistics");
+      auto num = [](double v) { return std::to_string(v); };
+      log_info("######################## Step " + std::to_string(done) + ", Elapsed time: " +
+               num(std::chrono::duration<double>(now - t_start).count()) +
+               " [sec] ########################");
+      log_info("Total kinetic energy         : " + num(st.kinetic_energy));
+      log_info("Max position                 : (" + num(st.max[0]) + ", " + num(st.max[1]) + ", " +
+               num(st.max[2]) + ")");
+      log_info("Min position                 : (" + num(st.min[0]) + ", " + num(st.min[1]) + ", " +
+               num(st.min[2]) + ")");
+      log_info("Number of collided particles : " + std::to_string(st.n_collided) + " / " +
+               std::to_string(n));
+      log_info("Max collision distance       : " + num(st.max_overlap));
       t_prev = now;
     }
   }

@@ -100,6 +100,7 @@ struct SimBase {
   virtual void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0,
                                double* gparams) = 0;
   virtual void window_stats(double* replay_ms, double* adjoint_ms) = 0;
+  virtual void statistics(psx_stats* out) = 0;
 };
 
 std::unique_ptr<SimBase> make_sim(int dtype);

@@ -887,6 +887,98 @@ __global__ void __launch_bounds__(1024)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Statistics of the current state (ParticleStatistics::calculateStatistics,
+// ParticleStatistics.cpp:3-43), on a bucketing of the current positions: kinetic energy
+// sum 0.5 m (v.v), the AABB of the spheres (x -/+ r), the number of particles i with an overlapping
+// partner j of higher index (r_i + r_j - |x_j - x_i| > 0) and the largest such overlap, all in
+// binary64. Contacts lie within the 27 cells around a particle's cell (the cell size is at least
+// 2 r_max), so the pair search is O(N). One partial record per workgroup; the host combines them in
+// workgroup order.
+// ---------------------------------------------------------------------------------------------
+template <class T, bool kTable>
+__global__ void __launch_bounds__(256)
+    k_stats(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
+            const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
+            const uint32_t* __restrict__ offset, const uint32_t* __restrict__ home,
+            StatPart* __restrict__ part) {
+  __shared__ double s_ke[256], s_ov[256], s_mn[3][256], s_mx[3][256];
+  __shared__ uint32_t s_nc[256];
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
+    const int32_t t = (int32_t)threadIdx.x;
+    const int32_t p = p0 + t;
+    double ke = 0.0, ov = 0.0;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t nc = 0;
+    if (p < p1) {
+      PairSelf<T> me;
+      uint32_t lo[9], hi[9];
+      pair_setup(&sc, pos, vel, offset, home, p, me, lo, hi);
+      const double xi[3] = {(double)me.x.x, (double)me.x.y, (double)me.x.z};
+      const double ri = (double)me.x.w;
+      const double vv = ((double)me.v.x * (double)me.v.x + (double)me.v.y * (double)me.v.y) +
+                        (double)me.v.z * (double)me.v.z;
+      ke = 0.5 * (double)me.v.w * vv;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = xi[a] - ri;
+        mx[a] = xi[a] + ri;
+      }
+      const int32_t mine = id[p];
+      bool hit = false;
+      for (int d = 0; d < 9; ++d)
+        for (uint32_t q = lo[d]; q < hi[d]; ++q) {
+          if (id[q] <= mine) continue;
+          const Vec4<T> xj = pos[q];
+          const double e0 = (double)xj.x - xi[0], e1 = (double)xj.y - xi[1],
+                       e2 = (double)xj.z - xi[2];
+          const double dist = sqrt((e0 * e0 + e1 * e1) + e2 * e2);  // glm::distance(x_i, x_j)
+          const double cd = ri + (double)xj.w - dist;
+          if (cd > 0.0) {
+            hit = true;
+            ov = fmax(ov, cd);
+          }
+        }
+      nc = hit ? 1u : 0u;
+    }
+    s_ke[t] = ke;
+    s_ov[t] = ov;
+    s_nc[t] = nc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s_mn[a][t] = mn[a];
+      s_mx[a][t] = mx[a];
+    }
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {  // fixed tree
+      if (t < w) {
+        s_ke[t] += s_ke[t + w];
+        s_ov[t] = fmax(s_ov[t], s_ov[t + w]);
+        s_nc[t] += s_nc[t + w];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          s_mn[a][t] = fmin(s_mn[a][t], s_mn[a][t + w]);
+          s_mx[a][t] = fmax(s_mx[a][t], s_mx[a][t + w]);
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      StatPart r;
+      r.ke = s_ke[0];
+      r.max_overlap = s_ov[0];
+      r.n_collided = s_nc[0];
+      r.count = (uint32_t)max(p1 - p0, 0);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        r.mn[a] = s_mn[a][0];
+        r.mx[a] = s_mx[a][0];
+      }
+      part[xcd_tile(blockIdx.x, loc.ntab)] = r;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
 // state in/out (original order <-> sorted order)
 // ---------------------------------------------------------------------------------------------
 template <class T>
@@ -1055,6 +1147,21 @@ hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const Bl
 }
 
 template <class T>
+hipError_t launch_stats(hipStream_t st, int ntab, const BlockInfo* btab, const SceneDev<T>* scenes,
+                        const SceneDev<T>* one, const Vec4<T>* pos, const Vec4<T>* vel,
+                        const int32_t* id, const uint32_t* offset, const uint32_t* home,
+                        StatPart* part) {
+  const Locator<T> loc = make_locator(ntab, 256, btab, scenes, one);
+  if (one)
+    hipLaunchKernelGGL((k_stats<T, false>), dim3(xcd_grid(ntab)), dim3(256), 0, st, loc, pos, vel,
+                       id, offset, home, part);
+  else
+    hipLaunchKernelGGL((k_stats<T, true>), dim3(xcd_grid(ntab)), dim3(256), 0, st, loc, pos, vel,
+                       id, offset, home, part);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_unsort(hipStream_t st, int32_t n, const Vec4<T>* pos, const Vec4<T>* vel,
                          const int32_t* id, T* pos3, T* vel3) {
   hipLaunchKernelGGL(k_unsort<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, pos, vel, id, pos3,
@@ -1089,6 +1196,10 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \
                                       Vec4<T>*);                                               \
+  template hipError_t launch_stats<T>(hipStream_t, int, const BlockInfo*, const SceneDev<T>*,    \
+                                      const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
+                                      const int32_t*, const uint32_t*, const uint32_t*,        \
+                                      StatPart*);                                              \
   template hipError_t launch_unsort<T>(hipStream_t, int32_t, const Vec4<T>*, const Vec4<T>*,    \
                                        const int32_t*, T*, T*);                                \
   template hipError_t launch_sort_in<T>(hipStream_t, int32_t, Vec4<T>*, Vec4<T>*,               \

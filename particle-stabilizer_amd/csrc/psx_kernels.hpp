@@ -87,6 +87,23 @@ hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const Bl
                         const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
                         Vec4<T>* vout);
 
+// Per-workgroup partial statistics (combined on the host in workgroup order).
+struct StatPart {
+  double ke;
+  double mn[3], mx[3];
+  double max_overlap;
+  uint32_t n_collided;
+  uint32_t count;  // particles of the workgroup (0 for an empty one)
+};
+
+// Statistics of the sorted state (pos, vel, id) bucketed by its current positions (offset, home
+// from that bucketing); ntab workgroups of 256 particles; part[ntab].
+template <class T>
+hipError_t launch_stats(hipStream_t st, int ntab, const BlockInfo* btab, const SceneDev<T>* scenes,
+                        const SceneDev<T>* one, const Vec4<T>* pos, const Vec4<T>* vel,
+                        const int32_t* id, const uint32_t* offset, const uint32_t* home,
+                        StatPart* part);
+
 template <class T>
 hipError_t launch_unsort(hipStream_t st, int32_t n, const Vec4<T>* pos, const Vec4<T>* vel,
                          const int32_t* id, T* pos3, T* vel3);

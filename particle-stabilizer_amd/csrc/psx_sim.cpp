@@ -274,7 +274,7 @@ struct SimImpl final : SimBase {
   // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
-  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, slot_of, list;
+  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, slot_of, list, stat_part;
   DevBuf count, offset, tile_state, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
@@ -968,6 +968,47 @@ struct SimImpl final : SimBase {
     if (perm) std::memcpy(perm, hperm.data(), (size_t)nn * 4);
   }
 
+  // ParticleStatistics::calculateStatistics (ParticleStatistics.cpp:3-43) per scene: bucketing of
+  // the current positions, per-workgroup partials on the device, combined here in workgroup order.
+  void statistics(psx_stats* out) override {
+    // bucket the current state; the cell-sorted copy lands in the spare buffers, which the next
+    // step overwrites, so the state itself is left as it is
+    bucket_and_sort(0, nullptr);
+    int nb;
+    const BlockInfo* bt = table(256, &nb);
+    stat_part.alloc((size_t)nb * sizeof(StatPart));
+    PSX_HIP(launch_stats<T>(stream, nb, single() ? nullptr : bt, scenes_d.as<SceneDev<T>>(),
+                            single() ? scenes_h.data() : nullptr, pos[1 - cur_pos].as<Vec4<T>>(),
+                            vel[1 - cur_vel].as<Vec4<T>>(), id[1 - cur_pos].as<int32_t>(),
+                            offset.as<uint32_t>(), home.as<uint32_t>(), stat_part.as<StatPart>()));
+    std::vector<StatPart> h((size_t)nb);
+    PSX_HIP(hipMemcpyAsync(h.data(), stat_part.p, h.size() * sizeof(StatPart), hipMemcpyDeviceToHost,
+                           stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+    size_t b = 0;  // workgroups follow the scenes in order, ceil(n_s / 256) each
+    for (size_t sc = 0; sc < scenes.size(); ++sc) {
+      psx_stats r;
+      r.kinetic_energy = 0.0;
+      r.n_collided = 0;
+      r.max_overlap = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        r.min[a] = INFINITY;
+        r.max[a] = -INFINITY;
+      }
+      for (int64_t k = 0; k < (scenes[sc].n + 255) / 256; ++k, ++b) {
+        const StatPart& q = h[b];
+        r.kinetic_energy += q.ke;
+        r.n_collided += q.n_collided;
+        r.max_overlap = std::max(r.max_overlap, q.max_overlap);
+        for (int a = 0; a < 3; ++a) {
+          r.min[a] = std::min(r.min[a], q.mn[a]);
+          r.max[a] = std::max(r.max[a], q.mx[a]);
+        }
+      }
+      out[sc] = r;
+    }
+  }
+
   uint32_t read_flags() override {
     uint32_t f[4];
     PSX_HIP(hipMemcpyAsync(f, flags.p, 16, hipMemcpyDeviceToHost, stream));
@@ -1309,6 +1350,14 @@ int psx_window_stats(const psx_sim* sim, double* replay_ms, double* adjoint_ms) 
   if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
   sim->impl->window_stats(replay_ms, adjoint_ms);
   return PSX_OK;
+}
+
+int psx_statistics(psx_sim* sim, psx_stats* out) {
+  if (!sim || !out) return set_error(PSX_ERR_INVALID, "NULL argument");
+  PSX_TRY({
+    sim->impl->statistics(out);
+    return PSX_OK;
+  })
 }
 
 int psx_profile(psx_sim* sim, int64_t nsteps, double* kernel_ms, double* launches_per_step,

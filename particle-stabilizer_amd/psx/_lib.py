@@ -42,6 +42,11 @@ class psx_scene(C.Structure):
     _fields_ = [("params", psx_params), ("mesh", psx_mesh), ("n", C.c_int64)]
 
 
+class psx_stats(C.Structure):
+    _fields_ = [("kinetic_energy", C.c_double), ("min", C.c_double * 3), ("max", C.c_double * 3),
+                ("n_collided", C.c_int64), ("max_overlap", C.c_double)]
+
+
 class psx_tuning(C.Structure):
     _fields_ = [("block_integrate", C.c_int32), ("block_sort", C.c_int32),
                 ("block_pairs", C.c_int32), ("use_graph", C.c_int32),
@@ -64,6 +69,10 @@ _SIGNATURES = {
     "psx_synchronize": (C.c_int, [C.c_void_p]),
     "psx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "psx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(psx_tuning)]),
+    "psx_resolve_overlaps": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "psx_write_obj_spheres": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "psx_statistics": (C.c_int, [C.c_void_p, C.c_void_p]),
     "psx_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(psx_tuning)]),
     "psx_get_state": (C.c_int, [C.c_void_p, _DP, _DP]),
     "psx_set_state": (C.c_int, [C.c_void_p, _DP, _DP]),

@@ -1,6 +1,7 @@
 """CSV particles and OBJ containers through the library's host-side readers and writers
 (psx_read_csv / psx_write_csv / psx_read_obj, include/psx.h), which follow ParticleIO.cpp:8-144
-and PolygonContainer::readFromFile (PolygonContainer.cpp:381-417)."""
+and PolygonContainer::readFromFile (PolygonContainer.cpp:381-417), and the post-run outputs
+(psx_resolve_overlaps, psx_write_obj_spheres: ParticleModel.cpp:154-252, ObjData.cpp:8-140)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -53,3 +54,29 @@ def read_obj(path: str | os.PathLike) -> np.ndarray:
     ptr = _DP()
     check(lib.psx_read_obj(os.fsencode(path), C.byref(nt), C.byref(ptr)))
     return _take(lib, ptr, 9 * nt.value, (nt.value, 9))
+
+
+def resolve_overlaps(position, radius, axis: int):
+    """ParticleModel::resolveOverlaps: returns (lifted positions, collided count, max overlap),
+    the count and maximum being the overlap check the reference logs afterwards."""
+    lib = _lib.load()
+    pos = np.array(position, np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(radius, np.float64).reshape(-1)
+    if r.shape[0] != pos.shape[0]:
+        raise ValueError("position and radius differ in length")
+    nc = C.c_int64()
+    mo = C.c_double()
+    check(lib.psx_resolve_overlaps(pos.shape[0], pos.ctypes.data_as(_DP), r.ctypes.data_as(_DP),
+                                   int(axis), C.byref(nc), C.byref(mo)))
+    return pos, nc.value, mo.value
+
+
+def write_obj_spheres(path: str | os.PathLike, position, radius) -> None:
+    """ObjData::write: every particle as a triangulated UV sphere in one OBJ file."""
+    lib = _lib.load()
+    pos = np.ascontiguousarray(position, np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(radius, np.float64).reshape(-1)
+    if r.shape[0] != pos.shape[0]:
+        raise ValueError("position and radius differ in length")
+    check(lib.psx_write_obj_spheres(os.fsencode(path), pos.shape[0], pos.ctypes.data_as(_DP),
+                                    r.ctypes.data_as(_DP)))

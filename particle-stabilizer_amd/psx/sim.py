@@ -232,6 +232,15 @@ class Simulation:
         check(self._lib.psx_window_stats(self._h, C.byref(a), C.byref(b)))
         return {"replay_ms": a.value, "adjoint_ms": b.value}
 
+    def statistics(self) -> list[dict]:
+        """ParticleStatistics::calculateStatistics of every scene (ParticleStatistics.cpp:3-43)."""
+        from ._lib import psx_stats
+        out = (psx_stats * self.num_scenes)()
+        check(self._lib.psx_statistics(self._h, C.cast(out, C.c_void_p)))
+        return [{"kinetic_energy": o.kinetic_energy, "min": np.array(o.min[:]),
+                 "max": np.array(o.max[:]), "n_collided": int(o.n_collided),
+                 "max_overlap": o.max_overlap} for o in out]
+
     def profile(self, nsteps: int) -> dict:
         """Mean device time (ms) per launch of each kernel over nsteps steps (HIP events)."""
         ms = np.zeros(32)
