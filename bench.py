@@ -277,6 +277,17 @@ def main() -> int:
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                 "traffic": measured_traffic(args.workload, dom), "avg_launch_ms": round(dom_ms, 5),
                 "algo_bytes_per_launch": algo[dom], "profiled_steps": psteps}
+    stats = None
+    if not args.no_extras:
+        # ParticleStatistics::calculateStatistics (ParticleStatistics.cpp:3-43) of the state, on
+        # the GPU grid: one bucketing pass plus the statistics kernel, results copied to the host
+        sim.statistics()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            st = sim.statistics()[0]
+        stats = {"ms_per_call": round((time.perf_counter() - t0) / 20 * 1e3, 4),
+                 "n_collided": st["n_collided"], "max_overlap": st["max_overlap"],
+                 "kinetic_energy": st["kinetic_energy"]}
     sim.close()
 
     line = {
@@ -300,6 +311,7 @@ def main() -> int:
             except Exception as e:  # report, keep the headline line
                 extras[name] = {"error": f"{type(e).__name__}: {e}"}
             print(f"bench.py: {name} done", file=sys.stderr, flush=True)
+        extras["statistics"] = stats
         line["extras"] = extras
     if not args.no_cpu_baseline:
         cpu_steps = args.cpu_steps or {"cylinder_settle": 20, "hopper_settle": 3,
