@@ -34,16 +34,22 @@ RETUNE = False
 def tuned(sim) -> dict:
     from psx.tune import autotune
     return autotune(sim, cache=TUNE_CACHE, force=RETUNE)["tuning"]
+
+
 METRIC = "particle-steps/s on 1 GPU (plain vs batched); adjoint/forward step time ratio"
 
 
-def algorithmic_bytes(n: int, cells: int, s: int) -> dict:
-    """Bytes each kernel must move per launch (each array touched once; s = scalar bytes)."""
+def algorithmic_bytes(n: int, cells: int, s: int, blocks: bool) -> dict:
+    """Bytes each kernel must move per launch (each array touched once; s = scalar bytes; blocks:
+    the per-cell id-block form of the cell lists, else the dense lists). In a run of steps the pair
+    kernel also does the next step's hash pass (ids read, cells written, counts updated, ids filed
+    into the id blocks); k_integrate only starts a run."""
     return {
         "k_integrate": n * (13 * s + 16),
-        "k_scan": 12 * cells,
+        "k_scan": (12 if blocks else 8) * cells,
+        "k_scatter_ids": 0 if blocks else 16 * n + 8 * cells,
         "k_rank_gather": n * (16 * s + 24),
-        "k_pairs": 11 * s * n + 8 * cells,
+        "k_pairs": 11 * s * n + 8 * cells + n * (12 if blocks else 8) + 8 * cells,
     }
 
 
@@ -263,14 +269,19 @@ def main() -> int:
     psteps = max(1, min(args.profile_steps, args.steps))
     prof = sim.profile(psteps)
     cells = sim.num_cells
-    algo = algorithmic_bytes(n, cells, s_bytes)
+    sv = tuning.get("sort_variant", 0)
+    blocks = sv == 1 or (sv == 0 and cells <= 1 << 18)
+    algo = algorithmic_bytes(n, cells, s_bytes, blocks)
     kernels = {}
     for name, d in prof.items():
+        if d["per_step"] == 0:
+            continue
         ms = d["ms"]
         b = algo.get(name, 0)
-        kernels[name] = {"ms": round(ms, 5), "algo_bytes": b,
+        kernels[name] = {"ms": round(ms, 5), "launches_per_step": round(d["per_step"], 4),
+                         "algo_bytes": b,
                          "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 and b else None}
-    dom = max(prof, key=lambda k: prof[k]["ms"])
+    dom = max(prof, key=lambda k: prof[k]["ms"] * prof[k]["per_step"])
     dom_ms = prof[dom]["ms"]
     achieved = algo[dom] / (dom_ms * 1e-3)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),

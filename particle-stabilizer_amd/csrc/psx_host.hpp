@@ -61,13 +61,14 @@ class Profiler {
   void next_step();
   hipEvent_t begin(int k);
   hipEvent_t end(int k);
+  // ms[k]: mean duration of kernel k's launches; per_step[k]: its launches per step
   void collect(double* ms, double* per_step);
 
  private:
   int nk_;
   int64_t step_ = 0;
   std::vector<hipEvent_t> ev_;
-  std::vector<char> used_ = std::vector<char>(64, 0);
+  std::vector<char> rec_;  // (step, kernel) pairs that were launched
 };
 
 struct SimBase {

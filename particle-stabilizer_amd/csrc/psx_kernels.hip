@@ -1,13 +1,15 @@
 // psx_kernels.hip -- the per-step kernels of the HIP backend (gfx950, wave64).
 //
 // One step (PhysicsEngine.cpp:38-48, CPU order: integrate, bucket from the pre-container positions,
-// container, pairs) is these launches on one stream:
+// container, pairs) is these launches on one stream (in a run of steps, k_integrate's work for the
+// next step is done by k_pairs at its end, so k_integrate only starts a run):
 //   k_integrate    x += v*dt; cell hash of the new (pre-container) x; per-cell count
 //                  (atomic, returns the particle's arrival slot), and the particle's id written to
 //                  that slot of its cell's id block (kCellIds slots; arrivals past them go to an
 //                  overflow list)
 //   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
-//                  look-back over tiles, wave64 scans inside a tile); clears the counts
+//                  look-back over tiles, wave64 scans inside a tile); clears the counts (id
+//                  blocks; with the dense lists k_scatter_ids counts them back down to zero)
 //   k_rank_gather  rank of each particle among its cell mates by id (stable order); x += v*dt;
 //                  v += g*dt (the same operations again); box walls or mesh contact; gather of the
 //                  result into cell-sorted order
@@ -302,8 +304,52 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
   v.z = v.z + s.gdt[2];
 }
 
+// ParticleModel::updateBuckets (ParticleModel.cpp:49-62) for particle p (id me) at its
+// pre-container position x: its cell and one more member counted there. With id blocks the
+// particle's arrival slot (the counter's old value) places its id in the cell's block (arrivals
+// past kCellIds go to the overflow list); for the dense lists the count alone is kept (an atomic
+// whose result is not waited for) and the scatter kernel takes the slots by counting back down.
+template <class T>
+__device__ __forceinline__ void bucket_entry(const SceneDev<T>& s, const Vec4<T>& x, int32_t me,
+                                             int32_t p, bool blocks, int32_t* __restrict__ cell_of,
+                                             uint32_t* __restrict__ count,
+                                             int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
+                                             uint32_t* __restrict__ ovf_ctr) {
+  const GridDev& g = s.grid;
+  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+  // the two branches start differently so that the compiler keeps the dense form's atomic without
+  // a return value
+  if (blocks) {
+    const uint32_t slot = atomicAdd(count + c, 1u);
+    cell_of[p] = c;
+    if (slot < (uint32_t)kCellIds) {
+      cell_ids[(size_t)c * kCellIds + slot] = me;
+    } else {  // at most n entries in all
+      const uint32_t o = atomicAdd(ovf_ctr, 1u);
+      ovf[o] = make_int2(c, me);
+    }
+  } else {
+    cell_of[p] = c;
+    atomicAdd(count + c, 1u);
+  }
+}
+
+// The pair kernel's hash pass for the next step (NextBucket): x + v_new dt from the snapshot
+// position and the new velocity, then the bucket entry, exactly as the hash kernel computes them
+// from the state the next step starts from.
+template <class T>
+__device__ __forceinline__ void next_bucket(const SceneDev<T>& s, const NextBucket& nb,
+                                            Vec4<T> x, const Vec4<T>& v, int32_t p) {
+  integrate_position(s, x, v);
+  bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids, nb.ovf,
+               nb.ovf_ctr);
+}
+
 // ---------------------------------------------------------------------------------------------
-// K1: integrate + cell hash + container + counts
+// K1: integrate + cell hash + counts
 // ---------------------------------------------------------------------------------------------
 template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(1024)
@@ -311,7 +357,7 @@ __global__ void __launch_bounds__(1024)
                 const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
-                uint32_t* __restrict__ ovf_ctr, uint32_t* __restrict__ slot_of, int mode) {
+                uint32_t* __restrict__ ovf_ctr, int mode) {
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     if (p >= p1) return;
@@ -321,36 +367,23 @@ __global__ void __launch_bounds__(1024)
       const Vec4<T> v = vel[p];
       integrate_position(s, x, v);
     }
-    // ParticleModel::updateBuckets (ParticleModel.cpp:49-62): the cell of the pre-container x.
-    const GridDev& g = s.grid;
-    const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-    const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-    const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-    const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
-    cell_of[p] = c;
-    const uint32_t slot = atomicAdd(count + c, 1u);
-    if constexpr (kBlocks) {
-      if (slot < (uint32_t)kCellIds) {
-        cell_ids[(size_t)c * kCellIds + slot] = me;
-      } else {  // at most n entries in all
-        const uint32_t o = atomicAdd(ovf_ctr, 1u);
-        ovf[o] = make_int2(c, me);
-      }
-    } else {
-      slot_of[p] = slot;  // the dense list is filled once the offsets are known
-    }
+    bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, ovf, ovf_ctr);
   });
 }
 
-// Dense cell lists (the scatter form of the bucketing): id to offset[cell] + arrival slot, an
-// unordered list per cell that the rank/gather kernel orders by id.
+// Dense cell lists (the scatter form of the bucketing): each particle takes a slot of its cell by
+// counting the cell's count back down (count - 1, ..., 0), and its id goes to offset[cell] + slot:
+// an unordered list per cell that the rank/gather kernel orders by id. The counts end at zero,
+// ready for the next hash pass.
 __global__ void __launch_bounds__(1024)
     k_scatter_ids(int32_t n, int32_t tiles, const int32_t* __restrict__ cell_of,
-                  const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ offset,
+                  uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
                   const int32_t* __restrict__ id, int32_t* __restrict__ list) {
   const int32_t p = xcd_tile(blockIdx.x, tiles) * (int32_t)blockDim.x + (int32_t)threadIdx.x;
   if (p >= n) return;
-  list[offset[cell_of[p]] + slot_of[p]] = id[p];
+  const int32_t c = cell_of[p];
+  const uint32_t slot = atomicSub(count + c, 1u) - 1u;
+  list[offset[c] + slot] = id[p];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -385,7 +418,7 @@ __device__ inline uint64_t tile_word(uint32_t value, uint64_t status, uint32_t e
 __global__ void __launch_bounds__(kScanThreads)
     k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
            uint64_t* __restrict__ tile_state, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
-           uint32_t* __restrict__ flags) {
+           uint32_t* __restrict__ flags, int32_t clear) {
   // A tile is kScanRounds sub-tiles of kScanThreads uint4 (4 cells each): thread t holds uint4
   // t of every sub-tile, so every load and store of a wave is contiguous.
   constexpr int kScanRounds = kScanPerThread / 4;
@@ -485,7 +518,7 @@ __global__ void __launch_bounds__(kScanThreads)
     run += u[q].z;
     o.w = run;
     dst[q * kScanThreads + threadIdx.x] = o;
-    if ((u[q].x | u[q].y | u[q].z | u[q].w) != 0u) src[q * kScanThreads + threadIdx.x] = zero;
+    if (clear && (u[q].x | u[q].y | u[q].z | u[q].w) != 0u) src[q * kScanThreads + threadIdx.x] = zero;
     round_base += rtot;
   }
   if (tile == ntiles - 1 && threadIdx.x == 0)
@@ -827,7 +860,7 @@ template <class T, bool kTable>
 __global__ void __launch_bounds__(1024)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
-            const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout) {
+            const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout, const NextBucket next) {
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     if (p >= p1) return;
@@ -836,7 +869,9 @@ __global__ void __launch_bounds__(1024)
     pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
     T ax = me.v.x, ay = me.v.y, az = me.v.z;
     pair_sum(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
-    vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+    const Vec4<T> vn{ax, ay, az, me.v.w};
+    vout[p] = vn;
+    if (next.count) next_bucket(sc, next, me.x, vn, p);
   });
 }
 
@@ -852,7 +887,8 @@ template <class T, int G, bool kTable>
 __global__ void __launch_bounds__(1024)
     k_pairs_lanes(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
                   const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
-                  const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout) {
+                  const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout,
+                  const NextBucket next) {
   static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "lanes per particle: 2, 4, 8 or 16");
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t g = (int32_t)threadIdx.x % G;
@@ -882,7 +918,11 @@ __global__ void __launch_bounds__(1024)
         }
       }
     }
-    if (g == 0) vout[p] = Vec4<T>{ax, ay, az, me.v.w};
+    if (g == 0) {
+      const Vec4<T> vn{ax, ay, az, me.v.w};
+      vout[p] = vn;
+      if (next.count) next_bucket(sc, next, me.x, vn, p);
+    }
   });
 }
 
@@ -1049,12 +1089,12 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, uint32_t* slot_of, int mode) {
+                            uint32_t* ovf_ctr, int mode) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, ovf, ovf_ctr, slot_of, mode)
+             count, cell_ids, ovf, ovf_ctr, mode)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1066,19 +1106,30 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* slot_of, const uint32_t* offset, const int32_t* id,
+                              uint32_t* count, const uint32_t* offset, const int32_t* id,
                               int32_t* list) {
   const int32_t tiles = (int32_t)((n + block - 1) / block);
-  PSX_LAUNCH(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of, slot_of,
+  PSX_LAUNCH(k_scatter_ids, dim3(xcd_grid(tiles)), dim3(block), 0, st, n, tiles, cell_of, count,
              offset, id, list);
   return hipGetLastError();
 }
 
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags) {
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear) {
   PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ntiles,
-             ovf_ctr, flags);
+             ovf_ctr, flags, clear ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1112,23 +1163,23 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
 template <class T, bool kTable>
 static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int block, int lanes,
                                  const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
-                                 const uint32_t* home, Vec4<T>* vout) {
+                                 const uint32_t* home, Vec4<T>* vout, const NextBucket& next) {
   const dim3 grid(xcd_grid(loc.ntab)), blk(block);
   switch (lanes) {
     case 1:
-      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
+      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     case 2:
-      PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     case 4:
-      PSX_LAUNCH((k_pairs_lanes<T, 4, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 4, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     case 8:
-      PSX_LAUNCH((k_pairs_lanes<T, 8, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 8, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     case 16:
-      PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout);
+      PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1140,10 +1191,10 @@ template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
                         const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
-                        Vec4<T>* vout) {
+                        Vec4<T>* vout, const NextBucket& next) {
   const Locator<T> loc = make_locator(ntab, block / lanes, btab, scenes, one);
-  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, home, vout);
-  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, home, vout);
+  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
+  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
 }
 
 template <class T>
@@ -1182,7 +1233,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, int2*, uint32_t*,     \
-                                          uint32_t*, int);                                     \
+                                          int);                                     \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
                                             const SceneDev<T>*, const SceneDev<T>*, bool,      \
                                             int32_t, const int32_t*, const uint32_t*,          \
@@ -1195,7 +1246,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \
-                                      Vec4<T>*);                                               \
+                                      Vec4<T>*, const NextBucket&);                            \
   template hipError_t launch_stats<T>(hipStream_t, int, const BlockInfo*, const SceneDev<T>*,    \
                                       const SceneDev<T>*, const Vec4<T>*, const Vec4<T>*,      \
                                       const int32_t*, const uint32_t*, const uint32_t*,        \

@@ -14,8 +14,8 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
-constexpr int kCellIds = 8;
-constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours  // id slots per cell; arrivals beyond go to the overflow list
+constexpr int kCellIds = 8;                   // id slots per cell; arrivals beyond go to the overflow list
+constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours
 // Largest workgroup of the rank/gather kernel: its binary64 form (rank, walls and the mesh contact
 // in double) needs more than the 128 registers a 1024-wide workgroup leaves per lane.
 template <class T>
@@ -32,33 +32,53 @@ constexpr int kListPad = 4;
 // reach three entries past the last particle).
 constexpr int kPairPad = 4;
 
+// The next step's hash pass (launch_integrate's work on a cell-sorted state), done by the pair kernel
+// for each particle right after its new velocity: x + v_new dt, the cell of that position, its
+// count and, with id blocks (blocks != 0), its entry in the cell's id block. The state the next
+// step starts from is the one the pair kernel reads and writes (sorted snapshot positions, new
+// velocities, sorted ids), so the result is what launch_integrate would produce from it.
+// count == nullptr: no hash pass.
+struct NextBucket {
+  const int32_t* id;
+  int32_t* cell_of;
+  uint32_t* count;
+  int32_t* cell_ids;
+  int2* ovf;
+  uint32_t* ovf_ctr;
+  int32_t blocks;
+};
+
 // Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
 // (ntab = workgroups); otherwise btab holds one (scene, slice) entry per workgroup.
 //
 // Hash and count: the cell of each particle's position after the integration step
-// (kModeIntegrate) or of its current position, and its arrival slot in the cell (atomic count).
-// blocks: the id is filed into the cell's id block (cell_ids: kCellIds slots per cell,
-// C_pad * kCellIds; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them
-// and must be 0 on entry, k_scan resets it). Otherwise the slot goes to slot_of and
-// launch_scatter_ids builds dense cell lists once the offsets are known.
+// (kModeIntegrate) or of its current position, counted in its cell (counts must be 0 on entry).
+// blocks: the id is filed into the cell's id block at its arrival slot (cell_ids: kCellIds slots
+// per cell, C_pad * kCellIds; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0]
+// counts them and must be 0 on entry, k_scan resets it). Otherwise launch_scatter_ids builds dense
+// cell lists once the offsets are known.
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, uint32_t* slot_of, int mode);
+                            uint32_t* ovf_ctr, int mode);
 
-// list[offset[cell_of[p]] + slot_of[p]] = id[p]: unordered dense cell lists.
+// p[0, n) = 0 (a kernel, so that it may sit in a captured stream).
+hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n);
+
+// list[offset[c] + slot] = id[p] with c = cell_of[p] and slot = --count[c]: unordered dense cell
+// lists; the counts end at zero.
 hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_t* cell_of,
-                              const uint32_t* slot_of, const uint32_t* offset, const int32_t* id,
+                              uint32_t* count, const uint32_t* offset, const int32_t* id,
                               int32_t* list);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
-// (the last entry is the total); clears the counts, moves ovf_ctr[0] to ovf_ctr[1] and zeroes
-// ovf_ctr[0]. tile_state (ntiles words) persists between launches and is zeroed once at
+// (the last entry is the total); clears the counts when `clear`, moves ovf_ctr[0] to ovf_ctr[1]
+// and zeroes ovf_ctr[0]. tile_state (ntiles words) persists between launches and is zeroed once at
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags);
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear);
 
 // Stable rank among cell mates (from the id blocks, or from the dense lists when !blocks); the
 // integration step (kModeIntegrate), container (kModeContainer:
@@ -85,7 +105,7 @@ template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
                         const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
                         const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
-                        Vec4<T>* vout);
+                        Vec4<T>* vout, const NextBucket& next);
 
 // Per-workgroup partial statistics (combined on the host in workgroup order).
 struct StatPart {

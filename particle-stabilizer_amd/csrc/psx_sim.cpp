@@ -274,7 +274,7 @@ struct SimImpl final : SimBase {
   // particle state, double-buffered; pos/id alternate every step, vel alternates only when a
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
-  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, slot_of, list, stat_part;
+  DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, list, stat_part;
   DevBuf count, offset, tile_state, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
@@ -459,7 +459,6 @@ struct SimImpl final : SimBase {
     }
     cell_of.alloc((size_t)nn * 4);
     home.alloc((size_t)nn * 4);
-    slot_of.alloc((size_t)nn * 4);
     list.alloc(((size_t)nn + kListPad) * 4);
     cell_ids.alloc((size_t)cpad * kCellIds * 4);
     ovf.alloc((size_t)nn * 8);
@@ -531,30 +530,45 @@ struct SimImpl final : SimBase {
     return cpad <= (int64_t(1) << 18);
   }
 
+  // Bucketing of a state: hash pass (k_integrate) then ordering (scan, lists, rank/gather).
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
               int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
               Vec4<T>* xpre_sorted = nullptr) {
-    int nb_i, nb_s;
+    hash(ps, vs, is, mode, prof);
+    order(ps, vs, is, pd, vd, idd, off, cell_sorted, mode, prof, xpre_sorted);
+  }
+
+  // Hash pass: cell, count and id-block entry or slot of every particle (counts must be zero).
+  void hash(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, int mode, Profiler* prof) {
+    int nb_i;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
+    const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
+    launch(0, prof, [&] {
+      return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
+                                 scenes_d.as<SceneDev<T>>(), one, id_blocks(), ps, vs, is,
+                                 cell_of.as<int32_t>(), count.as<uint32_t>(),
+                                 cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
+                                 mode);
+    });
+  }
+
+  // Ordering after a hash pass: offsets (clearing the counts), lists, rank/gather into pd/vd/idd.
+  void order(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
+             int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
+             Vec4<T>* xpre_sorted) {
+    int nb_s;
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
     const int32_t nn = (int32_t)n;
     const bool blocks = id_blocks();
-    launch(0, prof, [&] {
-      return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
-                                 scenes_d.as<SceneDev<T>>(), one, blocks, ps, vs, is,
-                                 cell_of.as<int32_t>(), count.as<uint32_t>(),
-                                 cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                 slot_of.as<uint32_t>(), mode);
-    });
     launch(1, prof, [&] {
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
-                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>());
+                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks);
     });
     if (!blocks)
       launch(4, prof, [&] {
         return launch_scatter_ids(stream, tuning.block_sort, nn, cell_of.as<int32_t>(),
-                                  slot_of.as<uint32_t>(), off, is, list.as<int32_t>());
+                                  count.as<uint32_t>(), off, is, list.as<int32_t>());
       });
     launch(2, prof, [&] {
       return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
@@ -582,23 +596,61 @@ struct SimImpl final : SimBase {
   int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
   int pair_particles_per_block() const { return tuning.block_pairs / pair_lanes(); }
 
+  // next_ids: with the sorted ids of the pair kernel's state, it also does the next step's hash
+  // pass (NextBucket); nullptr: velocities only.
   void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
-             Profiler* prof) {
+             Profiler* prof, const int32_t* next_ids = nullptr) {
     int nb_p;
     const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
+    NextBucket nb;
+    std::memset(&nb, 0, sizeof(nb));
+    if (next_ids) {
+      nb.id = next_ids;
+      nb.cell_of = cell_of.as<int32_t>();
+      nb.count = count.as<uint32_t>();
+      nb.cell_ids = cell_ids.as<int32_t>();
+      nb.ovf = ovf.as<int2>();
+      nb.ovf_ctr = ovf_ctr.as<uint32_t>();
+      nb.blocks = id_blocks() ? 1 : 0;
+    }
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
                              single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
                              single() ? scenes_h.data() : nullptr, ps, vs, off,
-                             home.as<uint32_t>(), vout);
+                             home.as<uint32_t>(), vout, nb);
     });
   }
 
+  // The hash pass of the current state is done (by k_integrate or by the previous step's pair
+  // kernel): the counts hold it and the next step skips k_integrate.
+  bool hashed = false;
+
+  void ensure_hashed(Profiler* prof) {
+    if (hashed) return;
+    hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
+         kModeIntegrate, prof);
+    hashed = true;
+  }
+
+  // Discards a pending hash pass (before the state, the parameters or the bucketing form change,
+  // or before a bucketing that starts from zero counts). A fill kernel, not a memset, in case the
+  // stream is being captured by the caller.
+  void drop_hash() {
+    if (!hashed) return;
+    PSX_HIP(launch_zero_u32(stream, count.as<uint32_t>(), (int64_t)cpad));
+    PSX_HIP(launch_zero_u32(stream, ovf_ctr.as<uint32_t>(), 1));
+    hashed = false;
+  }
+
+  // One step from a hashed state; leaves the next state hashed.
   void step_once(Profiler* prof) {
     const int a = cur_pos, vc = cur_vel;
-    bucket_and_sort(kModeIntegrate | kModeContainer | kModeCheck, prof);
+    ensure_hashed(prof);
+    order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
+          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr,
+          kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr);
     pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offset.as<uint32_t>(),
-          vel[vc].as<Vec4<T>>(), prof);
+          vel[vc].as<Vec4<T>>(), prof, id[1 - a].as<int32_t>());
     cur_pos = 1 - a;
   }
 
@@ -756,6 +808,7 @@ struct SimImpl final : SimBase {
   }
 
   void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0, double* gparams) override {
+    drop_hash();  // the recompute buckets from zero counts
     if (win_nseg <= 0) throw Error(PSX_ERR_STATE, "psx_window_backward needs a psx_window_forward first");
     const size_t nn = (size_t)n;
     PSX_HIP(hipMemcpyAsync(lxA.p, dlx, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
@@ -819,6 +872,7 @@ struct SimImpl final : SimBase {
         throw Error(PSX_ERR_INVALID, "psx_set_params cannot change the container bounds");
     if (p.dtype != sh.prm.dtype) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change dtype");
     sh.prm = p;
+    drop_hash();  // hashed with the old dt
     PSX_HIP(hipStreamSynchronize(stream));
     SceneDev<T>& d = scenes_h[(size_t)scene];
     fill_coefficients(d, sh.prm);
@@ -870,6 +924,7 @@ struct SimImpl final : SimBase {
 
   void step(int64_t nsteps) override {
     int64_t left = nsteps;
+    if (left > 0) ensure_hashed(nullptr);  // outside any capture: the graphs start hashed
     if (tuning.use_graph) {
       while (left >= kGraphSteps) {
         if (!graphs[cur_pos][cur_vel]) capture(cur_pos, cur_vel);
@@ -897,6 +952,7 @@ struct SimImpl final : SimBase {
   }
 
   void retune(const psx_tuning& t) override {
+    drop_hash();  // the bucketing form may change
     PSX_HIP(hipStreamSynchronize(stream));
     destroy_graphs();
     tuning = t;
@@ -928,6 +984,7 @@ struct SimImpl final : SimBase {
       h[(size_t)(3 * nn + k)] = hv ? (T)hv[k] : T(0);
     }
     T* d = io.as<T>();
+    drop_hash();
     PSX_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
     PSX_HIP(launch_sort_in<T>(stream, (int32_t)nn, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
                               id[cur_pos].as<int32_t>(), hp ? d : nullptr, hv ? d + 3 * nn : nullptr));
@@ -941,6 +998,7 @@ struct SimImpl final : SimBase {
   }
 
   void set_state_device(const void* dp, const void* dv) override {
+    drop_hash();
     PSX_HIP(launch_sort_in<T>(stream, (int32_t)n, pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(),
                               id[cur_pos].as<int32_t>(), static_cast<const T*>(dp),
                               static_cast<const T*>(dv)));
@@ -949,6 +1007,7 @@ struct SimImpl final : SimBase {
   void debug_bucketing(int32_t* cell, uint32_t* cnt, uint32_t* off, int32_t* perm) override {
     const int a = cur_pos, vc = cur_vel;
     const int64_t nn = n;
+    drop_hash();
     bucket_and_sort(0, nullptr);
     std::vector<int32_t> hc((size_t)nn), hid((size_t)nn), hperm((size_t)nn);
     std::vector<uint32_t> hoff((size_t)ncells + 1);
@@ -973,6 +1032,7 @@ struct SimImpl final : SimBase {
   void statistics(psx_stats* out) override {
     // bucket the current state; the cell-sorted copy lands in the spare buffers, which the next
     // step overwrites, so the state itself is left as it is
+    drop_hash();
     bucket_and_sort(0, nullptr);
     int nb;
     const BlockInfo* bt = table(256, &nb);
@@ -1032,6 +1092,7 @@ const char* kernel_name(int k) {
 // ---------------------------------------------------------------------------------------------
 Profiler::Profiler(int nk, int64_t nsteps) : nk_(nk) {
   ev_.resize((size_t)(nk * std::max<int64_t>(nsteps, 1) * 2), nullptr);
+  rec_.assign((size_t)(nk * std::max<int64_t>(nsteps, 1)), 0);
   for (auto& e : ev_) PSX_HIP(hipEventCreate(&e));
 }
 Profiler::~Profiler() {
@@ -1040,7 +1101,7 @@ Profiler::~Profiler() {
 }
 void Profiler::next_step() { ++step_; }
 hipEvent_t Profiler::begin(int k) {
-  used_[k] = true;
+  rec_[(size_t)((step_ - 1) * nk_ + k)] = 1;
   return ev_[(size_t)(((step_ - 1) * nk_ + k) * 2)];
 }
 hipEvent_t Profiler::end(int k) { return ev_[(size_t)(((step_ - 1) * nk_ + k) * 2 + 1)]; }
@@ -1048,16 +1109,15 @@ void Profiler::collect(double* ms, double* per_step) {
   for (int k = 0; k < nk_; ++k) {
     double tot = 0.0;
     int64_t cnt = 0;
-    if (used_[k]) {
-      for (int64_t s = 0; s < step_; ++s) {
-        float t = 0.0f;
-        PSX_HIP(hipEventElapsedTime(&t, ev_[(size_t)((s * nk_ + k) * 2)], ev_[(size_t)((s * nk_ + k) * 2 + 1)]));
-        tot += t;
-        ++cnt;
-      }
+    for (int64_t s = 0; s < step_; ++s) {
+      if (!rec_[(size_t)(s * nk_ + k)]) continue;
+      float t = 0.0f;
+      PSX_HIP(hipEventElapsedTime(&t, ev_[(size_t)((s * nk_ + k) * 2)], ev_[(size_t)((s * nk_ + k) * 2 + 1)]));
+      tot += t;
+      ++cnt;
     }
     ms[k] = cnt ? tot / (double)cnt : 0.0;
-    if (per_step) per_step[k] = used_[k] ? 1.0 : 0.0;
+    if (per_step) per_step[k] = step_ ? (double)cnt / (double)step_ : 0.0;
   }
 }
 
