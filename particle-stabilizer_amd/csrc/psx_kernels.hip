@@ -310,7 +310,7 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
 // past kCellIds go to the overflow list); for the dense lists the count alone is kept (an atomic
 // whose result is not waited for) and the scatter kernel takes the slots by counting back down.
 template <class T>
-__device__ __forceinline__ void bucket_entry(const SceneDev<T>& s, const Vec4<T>& x, int32_t me,
+__device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4<T>& x, int32_t me,
                                              int32_t p, bool blocks, int32_t* __restrict__ cell_of,
                                              uint32_t* __restrict__ count,
                                              int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
@@ -335,18 +335,68 @@ __device__ __forceinline__ void bucket_entry(const SceneDev<T>& s, const Vec4<T>
     cell_of[p] = c;
     atomicAdd(count + c, 1u);
   }
+  return c;
 }
 
 // The pair kernel's hash pass for the next step (NextBucket): x + v_new dt from the snapshot
 // position and the new velocity, then the bucket entry, exactly as the hash kernel computes them
-// from the state the next step starts from.
+// from the state the next step starts from. Returns the cell (tile counts are added per workgroup
+// by WgTiles).
 template <class T>
-__device__ __forceinline__ void next_bucket(const SceneDev<T>& s, const NextBucket& nb,
-                                            Vec4<T> x, const Vec4<T>& v, int32_t p) {
+__device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextBucket& nb,
+                                               Vec4<T> x, const Vec4<T>& v, int32_t p) {
   integrate_position(s, x, v);
-  bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids, nb.ovf,
-               nb.ovf_ctr);
+  return bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids,
+                      nb.ovf, nb.ovf_ctr);
 }
+
+// Tile counts of a hash-pass workgroup (TileCounts). Its particles are consecutive in cell order
+// and move little in a step, so their cells lie almost all in the tiles around the first
+// particle's cell: they are summed in LDS slots for the kWgTiles tiles from one below that tile
+// and added to the global counts once per slot; cells outside the slots (particles crossing into
+// a distant x or y column) go to the global counts directly. Every thread calls begin(), add()
+// (with c < 0 where it holds no particle) and end().
+constexpr int kWgTiles = 4;
+struct WgTiles {
+  uint32_t cnt[kWgTiles];
+  int32_t base;
+
+  // cell0: a cell of the workgroup's first particle (thread 0's value is used)
+  __device__ void begin(int32_t cell0) {
+    if (threadIdx.x < kWgTiles) cnt[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) base = (cell0 >> kDirectShift) - 1;
+    __syncthreads();
+  }
+  __device__ void put(uint32_t* __restrict__ tc, int32_t t, uint32_t k) {
+    const int32_t slot = t - base;
+    if (slot >= 0 && slot < kWgTiles) atomicAdd(&cnt[slot], k);
+    else atomicAdd(tc + (size_t)t * kTileStride, k);
+  }
+  // lanes with c >= 0 take part; one addition per wave when they share a tile
+  __device__ void add(const TileCounts& tiles, int32_t c) {
+    uint32_t* __restrict__ tc =
+        tiles.tc + (size_t)(*tiles.gen & 1u) * (size_t)tiles.ntiles * kTileStride;
+    const bool has = c >= 0;
+    const uint64_t act = __ballot(has);
+    if (!has) return;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int32_t t = c >> kDirectShift;
+    const int32_t t0 = __shfl(t, leader, 64);
+    if (__ballot(t == t0) == act) {
+      if (lane_id() == leader) put(tc, t0, (uint32_t)__popcll(act));
+    } else {
+      put(tc, t, 1u);
+    }
+  }
+  __device__ void end(const TileCounts& tiles) {
+    __syncthreads();
+    if (threadIdx.x < kWgTiles && cnt[threadIdx.x] != 0u) {
+      uint32_t* __restrict__ tc =
+          tiles.tc + (size_t)(*tiles.gen & 1u) * (size_t)tiles.ntiles * kTileStride;
+      atomicAdd(tc + (size_t)(base + (int32_t)threadIdx.x) * kTileStride, cnt[threadIdx.x]);
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------------------------
 // K1: integrate + cell hash + counts
@@ -357,17 +407,25 @@ __global__ void __launch_bounds__(1024)
                 const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
-                uint32_t* __restrict__ ovf_ctr, int mode) {
+                uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode) {
+  __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
-    if (p >= p1) return;
-    Vec4<T> x = pos[p];
-    const int32_t me = id[p];
-    if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25 (positions only: the hash needs them)
-      const Vec4<T> v = vel[p];
-      integrate_position(s, x, v);
+    int32_t c = -1;
+    if (p < p1) {
+      Vec4<T> x = pos[p];
+      const int32_t me = id[p];
+      if (mode & kModeIntegrate) {  // PhysicsEngine.cpp:22-25 (positions only: the hash needs them)
+        const Vec4<T> v = vel[p];
+        integrate_position(s, x, v);
+      }
+      c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, ovf, ovf_ctr);
     }
-    bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, ovf, ovf_ctr);
+    if (tiles.tc) {
+      wt.begin(c);
+      wt.add(tiles, c);
+      wt.end(tiles);
+    }
   });
 }
 
@@ -525,6 +583,64 @@ __global__ void __launch_bounds__(kScanThreads)
     offset[(size_t)ntiles * kScanTile] = round_base;  // total: the offset one past the last cell
 }
 
+// The direct scan (TileCounts): tile b's prefix is the sum of the per-tile counts of tiles
+// [0, b) (read in parallel by the workgroup's 256 threads), so the tiles run independently.
+constexpr int kDirectThreads = kDirectTile / 4;  // one uint4 of counts per thread
+
+__global__ void __launch_bounds__(kDirectThreads)
+    k_scan_direct(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
+                  const TileCounts tiles, uint32_t* __restrict__ ovf_ctr, int32_t clear) {
+  constexpr int kWaves = kDirectThreads / 64;
+  __shared__ uint32_t wsum[kWaves], wpre[kWaves];
+  const int32_t tile = (int32_t)blockIdx.x;
+  const int w = threadIdx.x >> 6;
+  const int lane = lane_id();
+  const uint32_t par = *tiles.gen & 1u;  // every tile reads the same generation
+  const uint32_t* __restrict__ tc = tiles.tc + (size_t)par * (size_t)tiles.ntiles * kTileStride;
+  if (tile == 0 && threadIdx.x == 0) {  // as in the look-back scan
+    ovf_ctr[1] = ovf_ctr[0];
+    ovf_ctr[0] = 0u;
+  }
+  uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kDirectTile);
+  const uint4 u = src[threadIdx.x];
+  uint32_t before = 0;  // this thread's share of the earlier tiles' counts
+  for (int32_t j = (int32_t)threadIdx.x; j < tile; j += kDirectThreads)
+    before += tc[(size_t)j * kTileStride];
+  const uint32_t tot = (u.x + u.y) + (u.z + u.w);
+  const uint32_t wi = wave_inclusive_scan(tot);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
+  if (lane == 63) wsum[w] = wi;
+  if (lane == 0) wpre[w] = before;
+  __syncthreads();
+  uint32_t run = wi - tot, total = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) {
+    run += wpre[k] + ((k < w) ? wsum[k] : 0u);
+    total += wsum[k];
+  }
+  uint4 o;
+  o.x = run;
+  run += u.x;
+  o.y = run;
+  run += u.y;
+  o.z = run;
+  run += u.z;
+  o.w = run;
+  reinterpret_cast<uint4*>(offset + (size_t)tile * kDirectTile)[threadIdx.x] = o;
+  if (clear && (u.x | u.y | u.z | u.w) != 0u) src[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+  if (threadIdx.x == 0) {
+    // this tile's count of the other generation, for the hash pass that follows this bucketing
+    tiles.tc[((size_t)(par ^ 1u) * (size_t)tiles.ntiles + tile) * kTileStride] = 0u;
+    if (tile == tiles.ntiles - 1) {
+      uint32_t pre = 0;
+#pragma unroll
+      for (int k = 0; k < kWaves; ++k) pre += wpre[k];
+      offset[(size_t)tiles.ntiles * kDirectTile] = pre + total;  // one past the last cell
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // K4: stable rank + gather. The reference appends ids to a cell in ascending order
 // (ParticleModel.cpp:55-61), so a particle's sorted slot is offset[cell] + the number of cell mates
@@ -544,8 +660,10 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
                   const uint8_t* __restrict__ cull, int32_t n, Vec4<T>* __restrict__ pos_o,
                   Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
                   int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o,
-                  Vec4<T>* __restrict__ xpre_o) {
+                  Vec4<T>* __restrict__ xpre_o, uint32_t* __restrict__ gen) {
   static_assert(kCellIds == 8, "the id block is read as two int4");
+  // the scan of this bucketing is complete: the next hash pass counts tiles for the next one
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(gen, 1u);
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     const bool active = p < p1;  // inactive lanes stay for the wave-wide mesh path
@@ -861,17 +979,26 @@ __global__ void __launch_bounds__(1024)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
             const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout, const NextBucket next) {
+  __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
-    if (p >= p1) return;
-    PairSelf<T> me;
-    uint32_t lo[9], hi[9];
-    pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
-    T ax = me.v.x, ay = me.v.y, az = me.v.z;
-    pair_sum(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
-    const Vec4<T> vn{ax, ay, az, me.v.w};
-    vout[p] = vn;
-    if (next.count) next_bucket(sc, next, me.x, vn, p);
+    const bool tiles = next.count && next.tiles.tc;
+    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    int32_t c = -1;
+    if (p < p1) {
+      PairSelf<T> me;
+      uint32_t lo[9], hi[9];
+      pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+      T ax = me.v.x, ay = me.v.y, az = me.v.z;
+      pair_sum(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
+      const Vec4<T> vn{ax, ay, az, me.v.w};
+      vout[p] = vn;
+      if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+    }
+    if (tiles) {
+      wt.add(next.tiles, c);
+      wt.end(next.tiles);
+    }
   });
 }
 
@@ -890,38 +1017,47 @@ __global__ void __launch_bounds__(1024)
                   const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout,
                   const NextBucket next) {
   static_assert(G >= 2 && G <= 16 && (64 % G) == 0, "lanes per particle: 2, 4, 8 or 16");
+  __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t g = (int32_t)threadIdx.x % G;
     const int32_t p = p0 + (int32_t)threadIdx.x / G;
-    if (p >= p1) return;  // whole groups leave together
-    PairSelf<T> me;
-    uint32_t lo[9], hi[9];
-    pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
-    const int gbase = lane_id() & ~(G - 1);
-    T ax = me.v.x, ay = me.v.y, az = me.v.z;
+    const bool tiles = next.count && next.tiles.tc;
+    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    int32_t c = -1;
+    if (p < p1) {  // whole groups take part or not
+      PairSelf<T> me;
+      uint32_t lo[9], hi[9];
+      pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+      const int gbase = lane_id() & ~(G - 1);
+      T ax = me.v.x, ay = me.v.y, az = me.v.z;
 #pragma unroll
-    for (int d = 0; d < 9; ++d) {
-      const uint32_t e = hi[d];
-      for (uint32_t q0 = lo[d]; q0 < e; q0 += G) {  // the same trip count on a whole group
-        const uint32_t q = q0 + (uint32_t)g;
-        T fx = T(0), fy = T(0), fz = T(0);
-        bool hit = false;
-        if (q < e && (int32_t)q != p) hit = pair_kick(me, pos[q], vsn, q, fx, fy, fz);
-        uint32_t mine = (uint32_t)(__ballot(hit) >> gbase) & ((1u << G) - 1u);
-        while (mine) {  // identical on every lane of the group
-          const int src = gbase + __ffs(mine) - 1;
-          mine &= mine - 1u;
-          const T kx = __shfl(fx, src, 64), ky = __shfl(fy, src, 64), kz = __shfl(fz, src, 64);
-          ax = ax + kx;
-          ay = ay + ky;
-          az = az + kz;
+      for (int d = 0; d < 9; ++d) {
+        const uint32_t e = hi[d];
+        for (uint32_t q0 = lo[d]; q0 < e; q0 += G) {  // the same trip count on a whole group
+          const uint32_t q = q0 + (uint32_t)g;
+          T fx = T(0), fy = T(0), fz = T(0);
+          bool hit = false;
+          if (q < e && (int32_t)q != p) hit = pair_kick(me, pos[q], vsn, q, fx, fy, fz);
+          uint32_t mine = (uint32_t)(__ballot(hit) >> gbase) & ((1u << G) - 1u);
+          while (mine) {  // identical on every lane of the group
+            const int src = gbase + __ffs(mine) - 1;
+            mine &= mine - 1u;
+            const T kx = __shfl(fx, src, 64), ky = __shfl(fy, src, 64), kz = __shfl(fz, src, 64);
+            ax = ax + kx;
+            ay = ay + ky;
+            az = az + kz;
+          }
         }
       }
+      if (g == 0) {
+        const Vec4<T> vn{ax, ay, az, me.v.w};
+        vout[p] = vn;
+        if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+      }
     }
-    if (g == 0) {
-      const Vec4<T> vn{ax, ay, az, me.v.w};
-      vout[p] = vn;
-      if (next.count) next_bucket(sc, next, me.x, vn, p);
+    if (tiles) {
+      wt.add(next.tiles, c);
+      wt.end(next.tiles);
     }
   });
 }
@@ -1089,12 +1225,12 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, int mode) {
+                            uint32_t* ovf_ctr, const TileCounts& tiles, int mode) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, ovf, ovf_ctr, mode)
+             count, cell_ids, ovf, ovf_ctr, tiles, mode)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1133,6 +1269,13 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
   return hipGetLastError();
 }
 
+hipError_t launch_scan_direct(hipStream_t st, uint32_t* count, uint32_t* offset,
+                              const TileCounts& tiles, uint32_t* ovf_ctr, bool clear) {
+  PSX_LAUNCH(k_scan_direct, dim3(tiles.ntiles), dim3(kDirectThreads), 0, st, count, offset, tiles,
+             ovf_ctr, clear ? 1 : 0);
+  return hipGetLastError();
+}
+
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
@@ -1142,13 +1285,14 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o) {
+                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o,
+                              uint32_t* gen) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
              cell_ids, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode, cull, n,  \
-             pos_o, vel_o, id_o, cell_o, home_o, xpre_o)
+             pos_o, vel_o, id_o, cell_o, home_o, xpre_o, gen)
   if (one) {
     if (blocks) PSX_RANK(false, true);
     else PSX_RANK(false, false);
@@ -1233,7 +1377,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, int2*, uint32_t*,     \
-                                          int);                                     \
+                                          const TileCounts&, int);                                     \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
                                             const SceneDev<T>*, const SceneDev<T>*, bool,      \
                                             int32_t, const int32_t*, const uint32_t*,          \
@@ -1242,7 +1386,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                             const int32_t*, const TriEntry<T>*,                \
                                             const int32_t*, uint32_t*, int, const uint8_t*,    \
                                             Vec4<T>*, Vec4<T>*, int32_t*, int32_t*,            \
-                                            uint32_t*, Vec4<T>*);                              \
+                                            uint32_t*, Vec4<T>*, uint32_t*);                              \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \

@@ -38,6 +38,24 @@ constexpr int kPairPad = 4;
 // step starts from is the one the pair kernel reads and writes (sorted snapshot positions, new
 // velocities, sorted ids), so the result is what launch_integrate would produce from it.
 // count == nullptr: no hash pass.
+// Per-tile cell counts for the direct scan (grids of at most kDirectTiles tiles of kDirectTile
+// cells): the hash pass adds each particle to its tile's count, one atomic per wave when a wave's
+// particles share a tile, so that the scan finds every tile's prefix by summing the counts of the
+// tiles before it instead of waiting on them. Two generations of counts alternate (tc + g * ntiles
+// for generation parity g = *gen & 1): a scan reads its generation's and clears its own entry of
+// the other's, which the next hash pass fills. Tile t's count is word t * kTileStride. *gen advances once per bucketing, in the rank/gather
+// kernel. tc == nullptr: the look-back scan (larger grids).
+constexpr int kDirectTile = 1024;
+constexpr int kDirectShift = 10;
+constexpr int kDirectTiles = 1024;
+constexpr int kTileStride = 32;  // words between tile counts: one 128-byte line each, so that the
+                                 // atomics of different tiles do not queue on one line
+struct TileCounts {
+  uint32_t* tc;
+  uint32_t* gen;
+  int32_t ntiles;
+};
+
 struct NextBucket {
   const int32_t* id;
   int32_t* cell_of;
@@ -46,6 +64,7 @@ struct NextBucket {
   int2* ovf;
   uint32_t* ovf_ctr;
   int32_t blocks;
+  TileCounts tiles;
 };
 
 // Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
@@ -62,7 +81,7 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, int mode);
+                            uint32_t* ovf_ctr, const TileCounts& tiles, int mode);
 
 // p[0, n) = 0 (a kernel, so that it may sit in a captured stream).
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n);
@@ -80,6 +99,11 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear);
 
+// The same scan over tiles.ntiles tiles of kDirectTile cells, each tile's prefix summed from the
+// per-tile counts of the hash pass (TileCounts); no tile waits for another.
+hipError_t launch_scan_direct(hipStream_t st, uint32_t* count, uint32_t* offset,
+                              const TileCounts& tiles, uint32_t* ovf_ctr, bool clear);
+
 // Stable rank among cell mates (from the id blocks, or from the dense lists when !blocks); the
 // integration step (kModeIntegrate), container (kModeContainer:
 // walls or mesh) and non-finite check (kModeCheck) on the state (pos, vel); gather into
@@ -87,7 +111,7 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 // position (home_o, for the pair kernel: home cell | kHomeCull when cull[home cell] is set) of
 // every sorted slot. cull: one byte per cell, nonzero where the pair kernel may cull the
 // neighbour cells by distance. xpre_o (optional): the integrated, pre-container position of every
-// sorted slot (recorded for the adjoint).
+// sorted slot (recorded for the adjoint). gen: the bucketing generation, advanced by one.
 template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
@@ -97,7 +121,7 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o);
+                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o, uint32_t* gen);
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
 // home: the home word of every sorted slot (from the rank/gather pass that produced pos).

@@ -275,6 +275,8 @@ struct SimImpl final : SimBase {
   // bucketing pass runs without the pair kernel
   DevBuf pos[2], vel[2], id[2];
   DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, list, stat_part;
+  DevBuf tile_count, scan_gen;  // direct scan: per-tile counts (two generations), generation
+  int32_t ntiles_direct = 0;    // tiles of the direct scan, 0: look-back scan
   DevBuf count, offset, tile_state, flags;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
@@ -450,6 +452,7 @@ struct SimImpl final : SimBase {
     ncells = cells;
     cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
     ntiles = (int32_t)(cpad / kScanTile);
+    ntiles_direct = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
 
     // device allocations
     for (int k = 0; k < 2; ++k) {
@@ -466,6 +469,8 @@ struct SimImpl final : SimBase {
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1) * 4);
     tile_state.alloc((size_t)ntiles * 8);
+    tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct * kTileStride, 1) * 4);
+    scan_gen.alloc(4);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
     scenes_d.alloc(sd.size() * sizeof(SceneDev<T>));
@@ -481,6 +486,8 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemset(count.p, 0, count.bytes));
     PSX_HIP(hipMemset(offset.p, 0, offset.bytes));
     PSX_HIP(hipMemset(tile_state.p, 0, tile_state.bytes));
+    PSX_HIP(hipMemset(tile_count.p, 0, tile_count.bytes));
+    PSX_HIP(hipMemset(scan_gen.p, 0, scan_gen.bytes));
     PSX_HIP(hipMemset(ovf_ctr.p, 0, ovf_ctr.bytes));
     PSX_HIP(hipMemset(cell_ids.p, 0, cell_ids.bytes));
     PSX_HIP(hipMemset(flags.p, 0, flags.bytes));
@@ -530,6 +537,14 @@ struct SimImpl final : SimBase {
     return cpad <= (int64_t(1) << 18);
   }
 
+  TileCounts tiles() const {
+    TileCounts t;
+    t.tc = ntiles_direct ? tile_count.as<uint32_t>() : nullptr;
+    t.gen = scan_gen.as<uint32_t>();
+    t.ntiles = ntiles_direct;
+    return t;
+  }
+
   // Bucketing of a state: hash pass (k_integrate) then ordering (scan, lists, rank/gather).
   void bucket(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
               int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
@@ -548,7 +563,7 @@ struct SimImpl final : SimBase {
                                  scenes_d.as<SceneDev<T>>(), one, id_blocks(), ps, vs, is,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
-                                 mode);
+                                 tiles(), mode);
     });
   }
 
@@ -562,6 +577,9 @@ struct SimImpl final : SimBase {
     const int32_t nn = (int32_t)n;
     const bool blocks = id_blocks();
     launch(1, prof, [&] {
+      if (ntiles_direct)
+        return launch_scan_direct(stream, count.as<uint32_t>(), off, tiles(), ovf_ctr.as<uint32_t>(),
+                                  blocks);
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
                          ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks);
     });
@@ -577,7 +595,8 @@ struct SimImpl final : SimBase {
                                    ovf.as<int2>(), ovf_ctr.as<uint32_t>(), list.as<int32_t>(), ps,
                                    vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
-                                   cell_sorted, home.as<uint32_t>(), xpre_sorted);
+                                   cell_sorted, home.as<uint32_t>(), xpre_sorted,
+                                   scan_gen.as<uint32_t>());
     });
   }
 
@@ -612,6 +631,7 @@ struct SimImpl final : SimBase {
       nb.ovf = ovf.as<int2>();
       nb.ovf_ctr = ovf_ctr.as<uint32_t>();
       nb.blocks = id_blocks() ? 1 : 0;
+      nb.tiles = tiles();
     }
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
@@ -639,6 +659,7 @@ struct SimImpl final : SimBase {
     if (!hashed) return;
     PSX_HIP(launch_zero_u32(stream, count.as<uint32_t>(), (int64_t)cpad));
     PSX_HIP(launch_zero_u32(stream, ovf_ctr.as<uint32_t>(), 1));
+    PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct * kTileStride));
     hashed = false;
   }
 
