@@ -14,6 +14,7 @@
 // sorted by home cell, so the transpose is exact even where container pushes make home and bucket
 // cells differ. No atomics: every particle writes only its own adjoint and parameter partials.
 #include "psx_adjoint.hpp"
+#include "psx_bucket.hpp"
 
 namespace psx {
 
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(256)
                const TriBox<T>* __restrict__ tbox, const int32_t* __restrict__ tcell_off,
                const T* __restrict__ lx, const T* __restrict__ lv, T* __restrict__ lx_o,
                T* __restrict__ lv_o, T* __restrict__ gk, T* __restrict__ ge,
-               T* __restrict__ gg) {
+               T* __restrict__ gg, const NextBucket hn, const Vec4<T>* __restrict__ hpos) {
   static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "lanes per particle must divide 64");
   const int32_t tb = xcd_tile_a(blockIdx.x, ntab);
   if (tb >= ntab) return;
@@ -125,23 +126,48 @@ __global__ void __launch_bounds__(256)
     const int32_t hy = axis_index((double)xi.y, grid.min[1], grid.h, grid.n[1]);
     const int32_t hz = axis_index((double)xi.z, grid.min[2], grid.h, grid.n[2]);
     column_slices(grid, hx, hy, hz, offS, lo, hi);
+    auto own = [&](const Vec4<T>& xj, const Vec4<T>& vj) {
+      const T dv[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
+      const T uv[3] = {vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+      PairVjp<T> o;
+      pair_vjp(dv, uv, T(2) * vj.w / (vi.w + vj.w), xi.w + xj.w, s->spring, s->es, li, o);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ax[c] += o.gd[c];
+        av[c] += o.gu[c];
+      }
+      pk += o.gk;
+      pe += o.ge;
+    };
+    // first round over all nine slices at once (lane g takes member g of each): independent loads
+    // of the candidates, then of the touching ones' velocities, instead of nine round trips
+    uint32_t hitA = 0;
+    Vec4<T> xa[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const uint32_t q = lo[d] + (uint32_t)g;
+      if (q < hi[d] && (int32_t)q != p) {
+        xa[d] = posS[q];
+        hitA |= 1u << d;
+      }
+    }
+#pragma unroll
     for (int d = 0; d < 9; ++d)
-      for (uint32_t q = lo[d] + (uint32_t)g; q < hi[d]; q += G) {
+      if ((hitA >> d & 1u) && !touching(xi, xa[d])) hitA &= ~(1u << d);
+    Vec4<T> va[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+      if (hitA >> d & 1u) va[d] = velS[lo[d] + (uint32_t)g];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+      if (hitA >> d & 1u) own(xa[d], va[d]);
+    // slices with more than G members: the rest in rounds of G
+    for (int d = 0; d < 9; ++d)
+      for (uint32_t q = lo[d] + (uint32_t)(g + G); q < hi[d]; q += G) {
         if ((int32_t)q == p) continue;
         const Vec4<T> xj = posS[q];
         if (!touching(xi, xj)) continue;
-        const Vec4<T> vj = velS[q];
-        const T dv[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
-        const T uv[3] = {vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
-        PairVjp<T> o;
-        pair_vjp(dv, uv, T(2) * vj.w / (vi.w + vj.w), xi.w + xj.w, s->spring, s->es, li, o);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          ax[c] += o.gd[c];
-          av[c] += o.gu[c];
-        }
-        pk += o.gk;
-        pe += o.ge;
+        own(xj, velS[q]);
       }
   }
   // (B) kicks i contributes to: particles whose home cell is around bucket(i), home-sorted
@@ -150,23 +176,48 @@ __global__ void __launch_bounds__(256)
     const int32_t bz = lc % grid.n[2], rest = lc / grid.n[2];
     const int32_t by = rest % grid.n[1], bx = rest / grid.n[1];
     column_slices(grid, bx, by, bz, offH, lo, hi);
+    auto other = [&](const Vec4<T>& xj, const Vec4<T>& vj, int64_t j) {
+      const T lj[3] = {lv[3 * j], lv[3 * j + 1], lv[3 * j + 2]};
+      const T dv[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+      const T uv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+      PairVjp<T> o;
+      pair_vjp(dv, uv, T(2) * vi.w / (vj.w + vi.w), xj.w + xi.w, s->spring, s->es, lj, o);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ax[c] -= o.gd[c];
+        av[c] -= o.gu[c];
+      }
+    };
+    // first round over all nine slices at once, as in (A)
+    uint32_t hitB = 0;
+    Vec4<T> xb[9];
+    int32_t jb[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const uint32_t q = lo[d] + (uint32_t)g;
+      if (q < hi[d]) {
+        jb[d] = idH[q];
+        xb[d] = posH[q];
+        hitB |= 1u << d;
+      }
+    }
+#pragma unroll
     for (int d = 0; d < 9; ++d)
-      for (uint32_t q = lo[d] + (uint32_t)g; q < hi[d]; q += G) {
+      if ((hitB >> d & 1u) && ((int64_t)jb[d] == me || !touching(xb[d], xi))) hitB &= ~(1u << d);
+    Vec4<T> vb[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+      if (hitB >> d & 1u) vb[d] = velH[lo[d] + (uint32_t)g];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+      if (hitB >> d & 1u) other(xb[d], vb[d], jb[d]);
+    for (int d = 0; d < 9; ++d)
+      for (uint32_t q = lo[d] + (uint32_t)(g + G); q < hi[d]; q += G) {
         const int64_t j = idH[q];
         if (j == me) continue;
         const Vec4<T> xj = posH[q];
         if (!touching(xj, xi)) continue;
-        const Vec4<T> vj = velH[q];
-        const T lj[3] = {lv[3 * j], lv[3 * j + 1], lv[3 * j + 2]};
-        const T dv[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-        const T uv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
-        PairVjp<T> o;
-        pair_vjp(dv, uv, T(2) * vi.w / (vj.w + vi.w), xj.w + xi.w, s->spring, s->es, lj, o);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          ax[c] -= o.gd[c];
-          av[c] -= o.gu[c];
-        }
+        other(xj, velH[q], j);
       }
   }
   // fixed-order combination of the group's partial sums (identical on every lane)
@@ -252,6 +303,12 @@ __global__ void __launch_bounds__(256)
     lx_o[3 * me + c] = lxs[c];
     lv_o[3 * me + c] = lvs[c] + dt * lxs[c];
   }
+  // the hash pass of the snapshot the next adjoint step buckets by home cell
+  if (hn.count) {
+    const int32_t c = bucket_entry(*s, hpos[p], hn.id[p], p, hn.blocks != 0, hn.cell_of, hn.count,
+                                   hn.cell_ids, hn.ovf, hn.ovf_ctr);
+    tile_count_wave(hn.tiles, c);
+  }
 }
 
 // Deterministic sum of three per-particle partial arrays (fixed order: strided lane sums, then a
@@ -295,10 +352,11 @@ hipError_t launch_adj_step(hipStream_t st, int ntab, const BlockInfo* btab,
                            const Vec4<T>* xpreS, const Vec4<T>* posH, const Vec4<T>* velH,
                            const int32_t* idH, const uint32_t* offH, const TriGeo<T>* tgeo,
                            const TriBox<T>* tbox, const int32_t* tcell_off, const T* lx,
-                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg) {
+                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg,
+                           const NextBucket& hn, const Vec4<T>* hpos) {
   hipLaunchKernelGGL((k_adj_step<T, kAdjLanes>), dim3(xgrid(ntab)), dim3(256), 0, st, ntab, btab,
                      scenes, posS, velS, idS, offS, cellS, xpreS, posH, velH, idH, offH, tgeo,
-                     tbox, tcell_off, lx, lv, lx_o, lv_o, gk, ge, gg);
+                     tbox, tcell_off, lx, lv, lx_o, lv_o, gk, ge, gg, hn, hpos);
   return hipGetLastError();
 }
 
@@ -341,7 +399,8 @@ hipError_t launch_fill(hipStream_t st, int64_t n, T* a, T value) {
                                          const uint32_t*, const int32_t*, const Vec4<T>*,        \
                                          const Vec4<T>*, const Vec4<T>*, const int32_t*,         \
                                          const uint32_t*, const TriGeo<T>*, const TriBox<T>*,    \
-                                         const int32_t*, const T*, const T*, T*, T*, T*, T*, T*); \
+                                         const int32_t*, const T*, const T*, T*, T*, T*, T*, T*, \
+                                         const NextBucket&, const Vec4<T>*); \
   template hipError_t launch_copy2<T>(hipStream_t, int64_t, const T*, const T*, T*, T*);         \
   template hipError_t launch_reduce3<T>(hipStream_t, int32_t, const T*, const T*, const T*,      \
                                         double*);                                                \

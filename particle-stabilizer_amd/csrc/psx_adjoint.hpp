@@ -1,7 +1,7 @@
 // psx_adjoint.hpp -- launchers of the reverse-mode kernels (psx_adjoint.hip).
 #pragma once
 
-#include "psx_device.hpp"
+#include "psx_kernels.hpp"
 
 namespace psx {
 
@@ -11,7 +11,9 @@ constexpr int kAdjPerBlock = 256 / kAdjLanes;
 
 // Adjoint of one step (pair, container and integrate terms) for the bucket-sorted snapshot of the
 // step: btab holds kAdjPerBlock particles per workgroup. xpreS: integrated, pre-container positions
-// of the snapshot's slots.
+// of the snapshot's slots. With hn.count set, the kernel also does the hash pass (cell, count, id
+// block, tile count) of the snapshot hpos (ids hn.id) that the next adjoint step buckets by home
+// cell.
 template <class T>
 hipError_t launch_adj_step(hipStream_t st, int ntab, const BlockInfo* btab,
                            const SceneDev<T>* scenes, const Vec4<T>* posS, const Vec4<T>* velS,
@@ -19,7 +21,8 @@ hipError_t launch_adj_step(hipStream_t st, int ntab, const BlockInfo* btab,
                            const Vec4<T>* xpreS, const Vec4<T>* posH, const Vec4<T>* velH,
                            const int32_t* idH, const uint32_t* offH, const TriGeo<T>* tgeo,
                            const TriBox<T>* tbox, const int32_t* tcell_off, const T* lx,
-                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg);
+                           const T* lv, T* lx_o, T* lv_o, T* gk, T* ge, T* gg,
+                           const NextBucket& hn, const Vec4<T>* hpos);
 
 // a_o[i] = a[i], b_o[i] = b[i] for i < n.
 template <class T>

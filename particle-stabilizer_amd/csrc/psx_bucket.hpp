@@ -1,0 +1,64 @@
+// psx_bucket.hpp -- device functions of the hash pass shared by the kernels of the forward step
+// (psx_kernels.hip) and of the adjoint (psx_adjoint.hip).
+#pragma once
+
+#include "psx_kernels.hpp"
+
+namespace psx {
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+// ParticleModel::updateBuckets (ParticleModel.cpp:49-62) for particle p (id me) at its
+// pre-container position x: its cell and one more member counted there. With id blocks the
+// particle's arrival slot (the counter's old value) places its id in the cell's block (arrivals
+// past kCellIds go to the overflow list); for the dense lists the count alone is kept (an atomic
+// whose result is not waited for) and the scatter kernel takes the slots by counting back down.
+template <class T>
+__device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4<T>& x,
+                                                int32_t me, int32_t p, bool blocks,
+                                                int32_t* __restrict__ cell_of,
+                                                uint32_t* __restrict__ count,
+                                                int32_t* __restrict__ cell_ids,
+                                                int2* __restrict__ ovf,
+                                                uint32_t* __restrict__ ovf_ctr) {
+  const GridDev& g = s.grid;
+  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+  // the two branches start differently so that the compiler keeps the dense form's atomic without
+  // a return value
+  if (blocks) {
+    const uint32_t slot = atomicAdd(count + c, 1u);
+    cell_of[p] = c;
+    if (slot < (uint32_t)kCellIds) {
+      cell_ids[(size_t)c * kCellIds + slot] = me;
+    } else {  // at most n entries in all
+      const uint32_t o = atomicAdd(ovf_ctr, 1u);
+      ovf[o] = make_int2(c, me);
+    }
+  } else {
+    cell_of[p] = c;
+    atomicAdd(count + c, 1u);
+  }
+  return c;
+}
+
+// Adds the calling lanes' particles (cells c) to their tiles' counts for the direct scan
+// (TileCounts): one atomic for the wave when its particles share a tile, else one per particle.
+__device__ __forceinline__ void tile_count_wave(const TileCounts& tiles, int32_t c) {
+  if (!tiles.tc) return;
+  uint32_t* __restrict__ tc =
+      tiles.tc + (size_t)(*tiles.gen & 1u) * (size_t)tiles.ntiles * kTileStride;
+  const int32_t t = c >> kDirectShift;
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll((unsigned long long)act) - 1;
+  const int32_t t0 = __shfl(t, leader, 64);
+  if (__ballot(t == t0) == act) {
+    if (lane_id() == leader) atomicAdd(tc + (size_t)t0 * kTileStride, (uint32_t)__popcll(act));
+  } else {
+    atomicAdd(tc + (size_t)t * kTileStride, 1u);
+  }
+}
+
+}  // namespace psx

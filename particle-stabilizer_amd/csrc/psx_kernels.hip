@@ -24,6 +24,7 @@
 
 #include <cstring>
 
+#include "psx_bucket.hpp"
 #include "psx_kernels.hpp"
 
 namespace psx {
@@ -61,7 +62,6 @@ __device__ inline int32_t xcd_tile(int32_t b, int32_t tiles) {
   return (b % kXcds) * per + b / kXcds;
 }
 
-__device__ inline int lane_id() { return threadIdx.x & 63; }
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = lane_id();
@@ -302,40 +302,6 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
   v.x = v.x + s.gdt[0];
   v.y = v.y + s.gdt[1];
   v.z = v.z + s.gdt[2];
-}
-
-// ParticleModel::updateBuckets (ParticleModel.cpp:49-62) for particle p (id me) at its
-// pre-container position x: its cell and one more member counted there. With id blocks the
-// particle's arrival slot (the counter's old value) places its id in the cell's block (arrivals
-// past kCellIds go to the overflow list); for the dense lists the count alone is kept (an atomic
-// whose result is not waited for) and the scatter kernel takes the slots by counting back down.
-template <class T>
-__device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4<T>& x, int32_t me,
-                                             int32_t p, bool blocks, int32_t* __restrict__ cell_of,
-                                             uint32_t* __restrict__ count,
-                                             int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
-                                             uint32_t* __restrict__ ovf_ctr) {
-  const GridDev& g = s.grid;
-  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
-  // the two branches start differently so that the compiler keeps the dense form's atomic without
-  // a return value
-  if (blocks) {
-    const uint32_t slot = atomicAdd(count + c, 1u);
-    cell_of[p] = c;
-    if (slot < (uint32_t)kCellIds) {
-      cell_ids[(size_t)c * kCellIds + slot] = me;
-    } else {  // at most n entries in all
-      const uint32_t o = atomicAdd(ovf_ctr, 1u);
-      ovf[o] = make_int2(c, me);
-    }
-  } else {
-    cell_of[p] = c;
-    atomicAdd(count + c, 1u);
-  }
-  return c;
 }
 
 // The pair kernel's hash pass for the next step (NextBucket): x + v_new dt from the snapshot

@@ -615,16 +615,13 @@ struct SimImpl final : SimBase {
   int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
   int pair_particles_per_block() const { return tuning.block_pairs / pair_lanes(); }
 
-  // next_ids: with the sorted ids of the pair kernel's state, it also does the next step's hash
-  // pass (NextBucket); nullptr: velocities only.
-  void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
-             Profiler* prof, const int32_t* next_ids = nullptr) {
-    int nb_p;
-    const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
+  // Arguments of a hash pass done inside another kernel, for the particles with ids `ids` (in the
+  // kernel's particle order); ids == nullptr: none.
+  NextBucket next_bucket_args(const int32_t* ids) const {
     NextBucket nb;
     std::memset(&nb, 0, sizeof(nb));
-    if (next_ids) {
-      nb.id = next_ids;
+    if (ids) {
+      nb.id = ids;
       nb.cell_of = cell_of.as<int32_t>();
       nb.count = count.as<uint32_t>();
       nb.cell_ids = cell_ids.as<int32_t>();
@@ -633,6 +630,16 @@ struct SimImpl final : SimBase {
       nb.blocks = id_blocks() ? 1 : 0;
       nb.tiles = tiles();
     }
+    return nb;
+  }
+
+  // next_ids: with the sorted ids of the pair kernel's state, it also does the next step's hash
+  // pass (NextBucket); nullptr: velocities only.
+  void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
+             Profiler* prof, const int32_t* next_ids = nullptr) {
+    int nb_p;
+    const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
+    const NextBucket nb = next_bucket_args(next_ids);
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
                              single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
@@ -756,17 +763,20 @@ struct SimImpl final : SimBase {
     return k == 0 ? seg_id.as<int32_t>() : rs_id.as<int32_t>() + (k - 1) * n;
   }
 
+  // The recompute: as a run of steps, the hash pass of step k + 1 is done by step k's pair kernel
+  // (its pre-step state is step k's snapshot, new velocities and ids); the last step leaves none
+  // pending.
   void replay_segment(int64_t ks) {
     const size_t nn = (size_t)n, nc = (size_t)cpad + 1;  // offset record stride
+    if (ks > 0) hash(pre_pos(0), pre_vel(0), pre_id(0), kModeIntegrate, nullptr);
     for (int64_t k = 0; k < ks; ++k) {
       Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
       uint32_t* so = rs_off.as<uint32_t>() + k * nc;
-      bucket(pre_pos(k), pre_vel(k), pre_id(k), sp, sv,
-             rs_id.as<int32_t>() + k * nn, so, rs_cell.as<int32_t>() + k * nn,
-             kModeIntegrate | kModeContainer | kModeCheck, nullptr,
-             rs_xpre.as<Vec4<T>>() + k * nn);
-      pairs(sp, sv, so, rs_vnew.as<Vec4<T>>() + k * nn, nullptr);
+      int32_t* si = rs_id.as<int32_t>() + k * nn;
+      order(pre_pos(k), pre_vel(k), pre_id(k), sp, sv, si, so, rs_cell.as<int32_t>() + k * nn,
+            kModeIntegrate | kModeContainer | kModeCheck, nullptr, rs_xpre.as<Vec4<T>>() + k * nn);
+      pairs(sp, sv, so, rs_vnew.as<Vec4<T>>() + k * nn, nullptr, k + 1 < ks ? si : nullptr);
     }
   }
 
@@ -777,20 +787,25 @@ struct SimImpl final : SimBase {
     int nb;
     const BlockInfo* bt = table(kAdjPerBlock, &nb);
     T *lx = lxA.as<T>(), *lv = lvA.as<T>(), *lx_o = lxB.as<T>(), *lv_o = lvB.as<T>();
+    // home-cell bucketing of each snapshot (no integration, no container: nothing is written to
+    // the snapshot); the hash pass of step k - 1's is done by step k's adjoint kernel
+    if (ks > 0)
+      hash(rs_pos.as<Vec4<T>>() + (ks - 1) * nn, rs_vel.as<Vec4<T>>() + (ks - 1) * nn,
+           rs_id.as<int32_t>() + (ks - 1) * nn, 0, nullptr);
     for (int64_t k = ks - 1; k >= 0; --k) {
       Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
       const int32_t* si = rs_id.as<int32_t>() + k * nn;
-      // home-cell bucketing of the snapshot (hash only: nothing is written to sp/sv)
-      bucket(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
-             offH.as<uint32_t>(), nullptr, 0, nullptr);
+      order(sp, sv, si, posH.as<Vec4<T>>(), velH.as<Vec4<T>>(), idH.as<int32_t>(),
+            offH.as<uint32_t>(), nullptr, 0, nullptr, nullptr);
+      const NextBucket hn = next_bucket_args(k > 0 ? si - nn : nullptr);
       PSX_HIP(launch_adj_step<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
                                  rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,
                                  rs_xpre.as<Vec4<T>>() + k * nn, posH.as<Vec4<T>>(),
                                  velH.as<Vec4<T>>(), idH.as<int32_t>(), offH.as<uint32_t>(),
                                  tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
                                  tcell_off_d.as<int32_t>(), lx, lv, lx_o, lv_o, gk.as<T>(),
-                                 ge.as<T>(), gg.as<T>()));
+                                 ge.as<T>(), gg.as<T>(), hn, k > 0 ? sp - nn : nullptr));
       std::swap(lx, lx_o);
       std::swap(lv, lv_o);
     }
