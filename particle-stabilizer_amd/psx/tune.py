@@ -61,6 +61,7 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
     p0, v0 = sim.state()
     n_half = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
     samples: dict[tuple, list] = {}
+    launches: dict[tuple, list] = {}  # (kernel, sort variant) -> launches per step
     # the binary64 rank/gather kernel takes workgroups of at most 512
     sort_blocks = BLOCKS if np.dtype(sim.dtype) == np.float32 else tuple(b for b in BLOCKS if b <= 512)
     for k in range(2 * n_half):  # first half per-cell id blocks, second half dense lists
@@ -72,45 +73,59 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
                        sort_variant=sv, use_graph=0)
         prof = sim.profile(steps)
         for name, d in prof.items():
+            # the pair kernel also does the next step's hash pass, whose cost depends on the
+            # cell-list form: its samples are kept per form
             shape = {"k_integrate": (bi, sv), "k_scan": (0, sv), "k_rank_gather": (bs, sv),
-                     "k_scatter_ids": (bs, sv), "k_pairs": (bp, g)}.get(name)
-            if shape is None or d.get("per_step", 1) == 0:
+                     "k_scatter_ids": (bs, sv), "k_pairs": (bp, g, sv)}.get(name)
+            per = d.get("per_step", 1)
+            if shape is None or per == 0:
                 continue
             samples.setdefault((name,) + shape, []).append(d["ms"])
+            launches.setdefault((name, sv), []).append(per)
 
     def med(name, *key):
         ms = samples.get((name,) + key)
         return sorted(ms)[len(ms) // 2] if ms else None
 
-    # per bucketing variant: fastest hash kernel + fastest gather (+ scatter) size + scan
+    def rate(name, sv):
+        """Launches per step. k_integrate only starts a run of steps (after the state or the
+        tuning changes, as at the start of each profiled round here): none in a steady run."""
+        if name == "k_integrate":
+            return 0.0
+        w = launches.get((name, sv))
+        return sum(w) / len(w) if w else 0.0
+
+    # per cell-list form: the fastest shape of each kernel, weighted by its launches per step
     plans = {}
     for sv in (1, 2):
         bi_ms = {bi: med("k_integrate", bi, sv) for bi in BLOCKS}
+        bi_ms = {b: t for b, t in bi_ms.items() if t is not None}
         bs_ms = {}
         for bs in sort_blocks:
             r = med("k_rank_gather", bs, sv)
             sc = med("k_scatter_ids", bs, sv) if sv == 2 else 0.0
             if r is not None and sc is not None:
-                bs_ms[bs] = r + sc
-        bi_ms = {b: t for b, t in bi_ms.items() if t is not None}
-        scan = med("k_scan", 0, sv) or 0.0
-        if bi_ms and bs_ms:
+                bs_ms[bs] = r * rate("k_rank_gather", sv) + sc * rate("k_scatter_ids", sv)
+        pair_sv = {(bp, g): med("k_pairs", bp, g, sv) for bp, g in PAIR_SHAPES}
+        pair_sv = {k: t for k, t in pair_sv.items() if t is not None}
+        scan = (med("k_scan", 0, sv) or 0.0) * rate("k_scan", sv)
+        if bi_ms and bs_ms and pair_sv:
             bi = min(bi_ms, key=bi_ms.get)
             bs = min(bs_ms, key=bs_ms.get)
-            plans[sv] = (bi_ms[bi] + bs_ms[bs] + scan, bi, bs)
+            bp, g = min(pair_sv, key=pair_sv.get)
+            total = (bi_ms[bi] * rate("k_integrate", sv) + bs_ms[bs] + scan
+                     + pair_sv[(bp, g)] * rate("k_pairs", sv))
+            plans[sv] = (total, bi, bs, bp, g, pair_sv[(bp, g)])
     sv_best = min(plans, key=lambda v: plans[v][0])
-    _, bi_best, bs_best = plans[sv_best]
-    pair_ms = {(bp, g): med("k_pairs", bp, g) for bp, g in PAIR_SHAPES}
-    pair_ms = {k: t for k, t in pair_ms.items() if t is not None}
-    bp_best, g_best = min(pair_ms, key=pair_ms.get)
+    _, bi_best, bs_best, bp_best, g_best, pair_best = plans[sv_best]
     tuning = {"block_integrate": int(bi_best), "block_sort": int(bs_best),
               "block_pairs": int(bp_best), "pair_variant": int(g_best),
               "sort_variant": int(sv_best), "use_graph": 1 if use_graph else 0}
     sim.set_tuning(**tuning)
     sim.set_state(p0, v0)
-    kernel_ms = {"bucketing_id_blocks": plans.get(1, (None,))[0],
-                 "bucketing_dense_lists": plans.get(2, (None,))[0],
-                 "k_pairs": pair_ms[(bp_best, g_best)]}
+    kernel_ms = {"step_id_blocks": plans.get(1, (None,))[0],
+                 "step_dense_lists": plans.get(2, (None,))[0],
+                 "k_pairs": pair_best}
     if path is not None:
         data = _load(path)
         data[key] = {"tuning": tuning, "kernel_ms": kernel_ms}

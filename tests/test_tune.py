@@ -51,15 +51,45 @@ class FakeSim:
 def test_picks_fastest_shapes_and_restores_state():
     sim = FakeSim()
     out = tune.autotune(sim, steps=2)
-    # dense lists: the hash kernel saves 0.004 ms and the scatter kernel costs 0.003 ms
+    # the hash kernel only starts a run: dense lists cost the scatter kernel's 0.003 ms per step
     assert out["tuning"] == {"block_integrate": 512, "block_sort": 128, "block_pairs": 256,
-                             "pair_variant": 4, "sort_variant": 2, "use_graph": 1}
+                             "pair_variant": 4, "sort_variant": 1, "use_graph": 1}
     assert sim.tuning["pair_variant"] == 4 and sim.tuning["use_graph"] == 1
     assert sim.restored is not None
     np.testing.assert_array_equal(sim.restored[0], sim.pos)
     np.testing.assert_array_equal(sim.restored[1], sim.vel)
     # every pair shape and every block size was timed
     assert sim.profiled == len(tune.PAIR_SHAPES) * 2 * 2
+
+
+def test_pair_kernel_cost_taken_per_cell_list_form():
+    """k_integrate only starts a run (few launches per step) and the pair kernel's time depends on
+    the cell-list form (it does the next step's hash pass): the plan weighs each kernel by its
+    launches per step and takes the pair kernel's time of the same form."""
+    class RunSim(FakeSim):
+        def profile(self, steps):
+            out = super().profile(steps)
+            dense = self.tuning["sort_variant"] == 2
+            out["k_integrate"]["per_step"] = 1.0 / steps
+            out["k_integrate"]["ms"] += 0.0 if dense else 0.05  # id blocks: a slow hash kernel
+            out["k_pairs"]["ms"] += 0.0 if not dense else 0.005  # dense: slower pair kernel
+            return out
+
+    out = tune.autotune(RunSim(), steps=100)
+    # id blocks: a slower hash kernel that a steady run does not launch, 0.008 ms less per step
+    # (0.005 pair kernel + 0.003 scatter kernel)
+    assert out["tuning"]["sort_variant"] == 1
+
+
+def test_dense_lists_win_on_a_cheaper_pair_kernel():
+    class DenseSim(FakeSim):
+        def profile(self, steps):
+            out = super().profile(steps)
+            if self.tuning["sort_variant"] != 2:
+                out["k_pairs"]["ms"] += 0.004  # id blocks: the hash pass waits on its atomics
+            return out
+
+    assert tune.autotune(DenseSim(), steps=4)["tuning"]["sort_variant"] == 2
 
 
 def test_cache_written_and_reused(tmp_path):
