@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256)
   // the hash pass of the snapshot the next adjoint step buckets by home cell
   if (hn.count) {
     const int32_t c = bucket_entry(*s, hpos[p], hn.id[p], p, hn.blocks != 0, hn.cell_of, hn.count,
-                                   hn.cell_ids, hn.ovf, hn.ovf_ctr);
+                                   hn.cell_ids, hn.id_stride, hn.ovf, hn.ovf_ctr);
     tile_count_wave(hn.tiles, c);
   }
 }

@@ -10,16 +10,18 @@ __device__ inline int lane_id() { return threadIdx.x & 63; }
 
 // ParticleModel::updateBuckets (ParticleModel.cpp:49-62) for particle p (id me) at its
 // pre-container position x: its cell and one more member counted there. With id blocks the
-// particle's arrival slot (the counter's old value) places its id in the cell's block (arrivals
-// past kCellIds go to the overflow list); for the dense lists the count alone is kept (an atomic
-// whose result is not waited for) and the scatter kernel takes the slots by counting back down.
+// particle's arrival slot (the counter's old value) places its id in the cell's id slots, slot k of
+// cell c at cell_ids[k * stride + c] (slot-major: the first arrivals of consecutive cells, which
+// consecutive particles are, lie side by side); arrivals past kCellIds go to the overflow list.
+// For the dense lists the count alone is kept (an atomic whose result is not waited for) and the
+// scatter kernel takes the slots by counting back down.
 template <class T>
 __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4<T>& x,
                                                 int32_t me, int32_t p, bool blocks,
                                                 int32_t* __restrict__ cell_of,
                                                 uint32_t* __restrict__ count,
                                                 int32_t* __restrict__ cell_ids,
-                                                int2* __restrict__ ovf,
+                                                uint32_t stride, int2* __restrict__ ovf,
                                                 uint32_t* __restrict__ ovf_ctr) {
   const GridDev& g = s.grid;
   const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
@@ -32,7 +34,7 @@ __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4
     const uint32_t slot = atomicAdd(count + c, 1u);
     cell_of[p] = c;
     if (slot < (uint32_t)kCellIds) {
-      cell_ids[(size_t)c * kCellIds + slot] = me;
+      cell_ids[(size_t)slot * stride + (uint32_t)c] = me;
     } else {  // at most n entries in all
       const uint32_t o = atomicAdd(ovf_ctr, 1u);
       ovf[o] = make_int2(c, me);

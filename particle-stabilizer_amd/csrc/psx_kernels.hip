@@ -5,11 +5,13 @@
 // next step is done by k_pairs at its end, so k_integrate only starts a run):
 //   k_integrate    x += v*dt; cell hash of the new (pre-container) x; per-cell count
 //                  (atomic, returns the particle's arrival slot), and the particle's id written to
-//                  that slot of its cell's id block (kCellIds slots; arrivals past them go to an
-//                  overflow list)
-//   k_scan         exclusive scan of the per-cell counts into offsets, one pass (decoupled
-//                  look-back over tiles, wave64 scans inside a tile); clears the counts (id
-//                  blocks; with the dense lists k_scatter_ids counts them back down to zero)
+//                  that slot of its cell's id slots (kCellIds slot-major arrays; arrivals past
+//                  them go to an overflow list)
+//   k_scan         exclusive scan of the per-cell counts into offsets, one pass, wave64 scans
+//                  inside a tile; each tile's prefix is summed from the per-tile counts of the
+//                  hash pass (k_scan_direct, grids up to 2^20 cells) or found by decoupled
+//                  look-back (k_scan); clears the counts (id blocks; with the dense lists
+//                  k_scatter_ids counts them back down to zero)
 //   k_rank_gather  rank of each particle among its cell mates by id (stable order); x += v*dt;
 //                  v += g*dt (the same operations again); box walls or mesh contact; gather of the
 //                  result into cell-sorted order
@@ -313,7 +315,7 @@ __device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextB
                                                Vec4<T> x, const Vec4<T>& v, int32_t p) {
   integrate_position(s, x, v);
   return bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids,
-                      nb.ovf, nb.ovf_ctr);
+                      nb.id_stride, nb.ovf, nb.ovf_ctr);
 }
 
 // Tile counts of a hash-pass workgroup (TileCounts). Its particles are consecutive in cell order
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(1024)
     k_integrate(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
                 const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
-                int32_t* __restrict__ cell_ids, int2* __restrict__ ovf,
+                int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
                 uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode) {
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__(1024)
         const Vec4<T> v = vel[p];
         integrate_position(s, x, v);
       }
-      c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, ovf, ovf_ctr);
+      c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
     }
     if (tiles.tc) {
       wt.begin(c);
@@ -610,14 +612,15 @@ __global__ void __launch_bounds__(kDirectThreads)
 // ---------------------------------------------------------------------------------------------
 // K4: stable rank + gather. The reference appends ids to a cell in ascending order
 // (ParticleModel.cpp:55-61), so a particle's sorted slot is offset[cell] + the number of cell mates
-// with a smaller id. The mates are read from the cell's id block (one 32-byte load of kCellIds
-// slots, the first count of which are this pass's) and, for a cell with more than kCellIds
+// with a smaller id. The mates are read from the cell's first min(count, kCellIds) id slots
+// (slot-major, none for a particle alone in its cell) and, for a cell with more than kCellIds
 // members, from the overflow list.
 // ---------------------------------------------------------------------------------------------
 template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(kMaxBlockSort<T>)
     k_rank_gather(const Locator<T> loc, const int32_t* __restrict__ cell_of,
                   const uint32_t* __restrict__ offset, const int32_t* __restrict__ cell_ids,
+                  uint32_t id_stride,
                   const int2* __restrict__ ovf, const uint32_t* __restrict__ ovf_ctr,
                   const int32_t* __restrict__ list, const Vec4<T>* __restrict__ pos,
                   const Vec4<T>* __restrict__ vel,
@@ -627,7 +630,6 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
                   Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
                   int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o,
                   Vec4<T>* __restrict__ xpre_o, uint32_t* __restrict__ gen) {
-  static_assert(kCellIds == 8, "the id block is read as two int4");
   // the scan of this bucketing is complete: the next hash pass counts tiles for the next one
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(gen, 1u);
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
@@ -659,11 +661,11 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       uint32_t rank = 0;
       if constexpr (kBlocks) {
         const uint32_t cnt = e - b;
-        const int4* blk = reinterpret_cast<const int4*>(cell_ids + (size_t)c * kCellIds);
-        const int4 m0 = blk[0], m1 = blk[1];
-        const int32_t mates[kCellIds] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+        if (cnt > 1u) {  // a particle alone in its cell has rank 0
 #pragma unroll
-        for (int k = 0; k < kCellIds; ++k) rank += ((uint32_t)k < cnt && mates[k] < me) ? 1u : 0u;
+          for (int k = 0; k < kCellIds; ++k)
+            if ((uint32_t)k < cnt) rank += (cell_ids[(size_t)k * id_stride + (uint32_t)c] < me) ? 1u : 0u;
+        }
         if (cnt > (uint32_t)kCellIds) {
           const uint32_t len = ovf_ctr[1];
           for (uint32_t o = 0; o < len; ++o) {
@@ -1190,13 +1192,14 @@ template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
-                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, const TileCounts& tiles, int mode) {
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
+                            uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
+                            const TileCounts& tiles, int mode) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, ovf, ovf_ctr, tiles, mode)
+             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1246,7 +1249,8 @@ template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                               int32_t n, const int32_t* cell_of, const uint32_t* offset,
-                              const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
+                              const int32_t* cell_ids, uint32_t id_stride, const int2* ovf,
+                              const uint32_t* ovf_ctr,
                               const int32_t* list, const Vec4<T>* pos, const Vec4<T>* vel,
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
@@ -1257,7 +1261,8 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
   const dim3 grid(xcd_grid(ntab)), blk(block);
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
-             cell_ids, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode, cull, n,  \
+             cell_ids, id_stride, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode,\
+             cull, n,                                                                          \
              pos_o, vel_o, id_o, cell_o, home_o, xpre_o, gen)
   if (one) {
     if (blocks) PSX_RANK(false, true);
@@ -1342,12 +1347,12 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
   template hipError_t launch_integrate<T>(hipStream_t, int, int, const BlockInfo*,             \
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
-                                          int32_t*, uint32_t*, int32_t*, int2*, uint32_t*,     \
+                                          int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
                                           const TileCounts&, int);                                     \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
                                             const SceneDev<T>*, const SceneDev<T>*, bool,      \
                                             int32_t, const int32_t*, const uint32_t*,          \
-                                            const int32_t*, const int2*, const uint32_t*,      \
+                                            const int32_t*, uint32_t, const int2*, const uint32_t*,      \
                                             const int32_t*, const Vec4<T>*, const Vec4<T>*,    \
                                             const int32_t*, const TriEntry<T>*,                \
                                             const int32_t*, uint32_t*, int, const uint8_t*,    \

@@ -61,6 +61,7 @@ struct NextBucket {
   int32_t* cell_of;
   uint32_t* count;
   int32_t* cell_ids;
+  uint32_t id_stride;  // cells per id slot (slot-major id blocks)
   int2* ovf;
   uint32_t* ovf_ctr;
   int32_t blocks;
@@ -72,16 +73,17 @@ struct NextBucket {
 //
 // Hash and count: the cell of each particle's position after the integration step
 // (kModeIntegrate) or of its current position, counted in its cell (counts must be 0 on entry).
-// blocks: the id is filed into the cell's id block at its arrival slot (cell_ids: kCellIds slots
-// per cell, C_pad * kCellIds; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0]
-// counts them and must be 0 on entry, k_scan resets it). Otherwise launch_scatter_ids builds dense
-// cell lists once the offsets are known.
+// blocks: the id is filed into the cell's id slots at its arrival slot (cell_ids: kCellIds slots
+// per cell, slot-major, slot k of cell c at k * id_stride + c with id_stride = C_pad; ovf: n entries
+// (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them and must be 0 on entry, k_scan
+// resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
-                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids, int2* ovf,
-                            uint32_t* ovf_ctr, const TileCounts& tiles, int mode);
+                            int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
+                            uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
+                            const TileCounts& tiles, int mode);
 
 // p[0, n) = 0 (a kernel, so that it may sit in a captured stream).
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n);
@@ -116,7 +118,8 @@ template <class T>
 hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                               const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                               int32_t n, const int32_t* cell_of, const uint32_t* offset,
-                              const int32_t* cell_ids, const int2* ovf, const uint32_t* ovf_ctr,
+                              const int32_t* cell_ids, uint32_t id_stride, const int2* ovf,
+                              const uint32_t* ovf_ctr,
                               const int32_t* list, const Vec4<T>* pos, const Vec4<T>* vel,
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,

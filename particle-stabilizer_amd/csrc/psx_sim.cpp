@@ -562,7 +562,8 @@ struct SimImpl final : SimBase {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
                                  scenes_d.as<SceneDev<T>>(), one, id_blocks(), ps, vs, is,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
-                                 cell_ids.as<int32_t>(), ovf.as<int2>(), ovf_ctr.as<uint32_t>(),
+                                 cell_ids.as<int32_t>(), (uint32_t)cpad, ovf.as<int2>(),
+                                 ovf_ctr.as<uint32_t>(),
                                  tiles(), mode);
     });
   }
@@ -591,7 +592,7 @@ struct SimImpl final : SimBase {
     launch(2, prof, [&] {
       return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
                                    scenes_d.as<SceneDev<T>>(), one, blocks, nn,
-                                   cell_of.as<int32_t>(), off, cell_ids.as<int32_t>(),
+                                   cell_of.as<int32_t>(), off, cell_ids.as<int32_t>(), (uint32_t)cpad,
                                    ovf.as<int2>(), ovf_ctr.as<uint32_t>(), list.as<int32_t>(), ps,
                                    vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
@@ -625,6 +626,7 @@ struct SimImpl final : SimBase {
       nb.cell_of = cell_of.as<int32_t>();
       nb.count = count.as<uint32_t>();
       nb.cell_ids = cell_ids.as<int32_t>();
+      nb.id_stride = (uint32_t)cpad;
       nb.ovf = ovf.as<int2>();
       nb.ovf_ctr = ovf_ctr.as<uint32_t>();
       nb.blocks = id_blocks() ? 1 : 0;
