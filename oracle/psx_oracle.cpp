@@ -134,6 +134,69 @@ struct Mesh {
   int64_t nt = 0;
   std::vector<V3<T>> a, b, c, n, lo, hi;
   double min[3], max[3];
+  // Optional index for long test runs: the triangles whose AABB touches each cell of a uniform
+  // grid over the mesh bounds (cell of a coordinate: floor((x - org) / cs), clamped). It only skips
+  // triangles whose AABB is disjoint from the sphere's, which the loop below rejects anyway, so the
+  // results are those of the full loop.
+  bool indexed = false;
+  double org[3], cs = 1.0;
+  int64_t dims[3] = {1, 1, 1};
+  std::vector<int64_t> cell_off;
+  std::vector<int32_t> cell_tris;
+
+  int64_t axis_cell(double x, int a) const {
+    const double q = std::floor((x - org[a]) / cs);
+    return q < 0.0 ? 0 : (q >= (double)(dims[a] - 1) ? dims[a] - 1 : (int64_t)q);
+  }
+  void build_index() {
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      org[a] = min[a];
+      ext = std::max(ext, max[a] - min[a]);
+    }
+    cs = std::max(ext / 64.0, 1e-9);
+    for (int a = 0; a < 3; ++a) dims[a] = (int64_t)((max[a] - min[a]) / cs) + 1;
+    const int64_t ncell = dims[0] * dims[1] * dims[2];
+    std::vector<std::vector<int32_t>> lists((size_t)ncell);
+    for (int64_t t = 0; t < nt; ++t) {
+      int64_t l[3], h[3];
+      const double tl[3] = {(double)lo[t].x, (double)lo[t].y, (double)lo[t].z};
+      const double th[3] = {(double)hi[t].x, (double)hi[t].y, (double)hi[t].z};
+      for (int a = 0; a < 3; ++a) {
+        l[a] = axis_cell(tl[a], a);
+        h[a] = axis_cell(th[a], a);
+      }
+      for (int64_t i = l[0]; i <= h[0]; ++i)
+        for (int64_t j = l[1]; j <= h[1]; ++j)
+          for (int64_t k = l[2]; k <= h[2]; ++k)
+            lists[(size_t)((i * dims[1] + j) * dims[2] + k)].push_back((int32_t)t);
+    }
+    cell_off.assign((size_t)ncell + 1, 0);
+    for (int64_t c = 0; c < ncell; ++c) cell_off[(size_t)c + 1] = cell_off[(size_t)c] + (int64_t)lists[(size_t)c].size();
+    cell_tris.clear();
+    for (auto& l : lists) cell_tris.insert(cell_tris.end(), l.begin(), l.end());
+    indexed = true;
+  }
+  // ascending ids of the triangles whose AABB may touch [l, h]
+  void candidates(const V3<T>& l, const V3<T>& h, std::vector<int32_t>& out) const {
+    out.clear();
+    int64_t a0[3], a1[3];
+    const double dl[3] = {(double)l.x, (double)l.y, (double)l.z};
+    const double dh[3] = {(double)h.x, (double)h.y, (double)h.z};
+    for (int a = 0; a < 3; ++a) {
+      a0[a] = axis_cell(dl[a], a);
+      a1[a] = axis_cell(dh[a], a);
+    }
+    for (int64_t i = a0[0]; i <= a1[0]; ++i)
+      for (int64_t j = a0[1]; j <= a1[1]; ++j)
+        for (int64_t k = a0[2]; k <= a1[2]; ++k) {
+          const int64_t c = (i * dims[1] + j) * dims[2] + k;
+          out.insert(out.end(), cell_tris.begin() + cell_off[(size_t)c],
+                     cell_tris.begin() + cell_off[(size_t)c + 1]);
+        }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+  }
 };
 
 // PolygonContainer ctor (PolygonContainer.cpp:52-89) and bounds (288-318). Normals are evaluated in
@@ -241,6 +304,10 @@ struct Sim {
   Grid g;
   const Mesh<T>* mesh = nullptr;
   std::vector<std::vector<int64_t>> buckets;
+  // fast (test runs): the pair loop's particles are split among threads (each particle's sum is
+  // its own, read from the snapshot, so the results are those of the serial loop)
+  bool fast = false;
+  int nthreads = 1;
   // scratch
   std::vector<V3<T>> sx, sv, bv, ybuf, accbuf;
   std::vector<int64_t> counter;
@@ -309,7 +376,12 @@ void mesh_contact(Sim<T>& s, int nthreads) {  // PolygonContainer.cpp:237-286
   s.accbuf.assign(n, V3<T>{0, 0, 0});    // clearParticleBufferVelocity
   s.counter.assign(n, 0);                // clearIntersectCounter
 #ifdef _OPENMP
-#pragma omp parallel for num_threads(nthreads) schedule(static)
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+  std::vector<int32_t> cand;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
 #endif
   for (int64_t i = 0; i < n; ++i) {
     const V3<T> x = s.x[i];
@@ -320,7 +392,10 @@ void mesh_contact(Sim<T>& s, int nthreads) {  // PolygonContainer.cpp:237-286
     V3<T> y = s.ybuf[i];
     V3<T> acc = s.accbuf[i];
     int64_t cnt = 0;
-    for (int64_t t = 0; t < m.nt; ++t) {
+    if (m.indexed) m.candidates(lo, hi, cand);
+    const int64_t ncand = m.indexed ? (int64_t)cand.size() : m.nt;
+    for (int64_t k = 0; k < ncand; ++k) {
+      const int64_t t = m.indexed ? (int64_t)cand[(size_t)k] : k;
       if (lo.x > m.hi[t].x || hi.x < m.lo[t].x || lo.y > m.hi[t].y || hi.y < m.lo[t].y ||
           lo.z > m.hi[t].z || hi.z < m.lo[t].z)
         continue;
@@ -341,6 +416,7 @@ void mesh_contact(Sim<T>& s, int nthreads) {  // PolygonContainer.cpp:237-286
     s.accbuf[i] = acc;
     s.counter[i] = cnt;
   }
+  }
   for (int64_t i = 0; i < n; ++i) {
     if (s.counter[i] > 0) {
       s.x[i] = s.ybuf[i];
@@ -356,6 +432,9 @@ void pair_contact(Sim<T>& s) {  // ParticleModel.cpp:64-144
   s.sv = s.v;
   s.bv = s.v;
   const T k = s.p.spring, es = s.p.es;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(s.nthreads) schedule(static) if (s.fast)
+#endif
   for (int64_t i = 0; i < n; ++i) {
     int64_t hc[3];
     cell3(s.g, s.sx[i], hc);
@@ -421,7 +500,7 @@ Params<T> make_params(const double* p) {
 
 template <class T>
 int run(const double* params, int64_t n, T* pos, T* vel, const T* r, const T* m, int64_t ntri,
-        const double* tri, int64_t nsteps, int nthreads) {
+        const double* tri, int64_t nsteps, int nthreads, int fast) {
   if (n <= 0 || !params || !pos || !vel || !r || !m) return 1;
   Sim<T> s;
   s.n = n;
@@ -434,10 +513,13 @@ int run(const double* params, int64_t n, T* pos, T* vel, const T* r, const T* m,
     s.v[i] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
   }
   s.p = make_params<T>(params);
+  s.fast = fast != 0;
+  s.nthreads = std::max(nthreads, 1);
   Mesh<T> mesh;
   double mn[3], mx[3];
   if (ntri > 0) {
     mesh = make_mesh<T>(ntri, tri);
+    if (fast) mesh.build_index();
     s.mesh = &mesh;
     for (int k = 0; k < 3; ++k) {
       mn[k] = mesh.min[k];
@@ -508,15 +590,19 @@ int64_t oracle_axis_index(double x, double mn, double h, int64_t n) {
 // Full runs. params = gravity[3], walls[6], spring, restitution, dt, box_min[3], box_max[3]
 // (18 doubles). tri: 9*ntri double vertices (may be NULL when ntri == 0). pos/vel are updated in
 // place (n x 3).
+// fast != 0 (long test runs): the mesh loop visits only the triangles a triangle-grid index
+// offers and the pair loop runs on nthreads threads; both give the results of the reference's
+// loops. 0: every triangle and a serial pair loop, as the reference runs them.
 int oracle_run_f64(const double* params, int64_t n, double* pos, double* vel, const double* r,
                    const double* m, int64_t ntri, const double* tri, int64_t nsteps,
-                   int nthreads) {
-  return run<double>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads);
+                   int nthreads, int fast) {
+  return run<double>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, fast);
 }
 
 int oracle_run_f32(const double* params, int64_t n, float* pos, float* vel, const float* r,
-                   const float* m, int64_t ntri, const double* tri, int64_t nsteps, int nthreads) {
-  return run<float>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads);
+                   const float* m, int64_t ntri, const double* tri, int64_t nsteps, int nthreads,
+                   int fast) {
+  return run<float>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, fast);
 }
 
 int oracle_bucketing_f64(double rmax, const double* mn, const double* mx, int64_t n,

@@ -39,9 +39,9 @@ def load() -> C.CDLL:
         lib.oracle_axis_index.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int64]
         lib.oracle_axis_index.restype = C.c_int64
         lib.oracle_run_f64.argtypes = [_DP, C.c_int64, _DP, _DP, _DP, _DP, C.c_int64, _DP,
-                                       C.c_int64, C.c_int]
+                                       C.c_int64, C.c_int, C.c_int]
         lib.oracle_run_f32.argtypes = [_DP, C.c_int64, _FP, _FP, _FP, _FP, C.c_int64, _DP,
-                                       C.c_int64, C.c_int]
+                                       C.c_int64, C.c_int, C.c_int]
         lib.oracle_bucketing_f64.argtypes = [C.c_double, _DP, _DP, C.c_int64, _DP, _I32P, _U32P,
                                              _U32P, _I32P]
         lib.oracle_bucketing_f32.argtypes = [C.c_double, _DP, _DP, C.c_int64, _FP, _I32P, _U32P,
@@ -76,8 +76,11 @@ def axis_index(x: float, mn: float, h: float, n: int) -> int:
 
 
 def run(params: np.ndarray, pos, vel, radius, mass, nsteps: int, mesh=None, dtype="f64",
-        nthreads: int | None = None):
-    """Runs nsteps reference steps; returns (pos, vel) in `dtype`."""
+        nthreads: int | None = None, fast: bool = False):
+    """Runs nsteps reference steps; returns (pos, vel) in `dtype`. fast: the mesh loop visits only
+    the triangles a triangle-grid index offers and the pair loop is split among threads (the
+    results of the reference's loops, much sooner); by default every triangle and a serial pair
+    loop, as the reference runs them."""
     lib = load()
     t = np.float64 if dtype in ("f64", np.float64) else np.float32
     ct = C.c_double if t is np.float64 else C.c_float
@@ -96,7 +99,8 @@ def run(params: np.ndarray, pos, vel, radius, mass, nsteps: int, mesh=None, dtyp
     fn = lib.oracle_run_f64 if t is np.float64 else lib.oracle_run_f32
     nthreads = nthreads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or lib.oracle_max_threads()
     rc = fn(_p(params, C.c_double), n, _p(pos, ct), _p(vel, ct), _p(r, ct), _p(m, ct), ntri,
-            _p(tri, C.c_double) if tri is not None else None, int(nsteps), int(nthreads))
+            _p(tri, C.c_double) if tri is not None else None, int(nsteps), int(nthreads),
+            1 if fast else 0)
     if rc != 0:
         raise RuntimeError(f"oracle run failed ({rc})")
     return pos, vel

@@ -361,3 +361,29 @@ def test_mass_formula(oracle):
         m = oracle.mass(1.0, r)
         assert math.isclose(m, 4.0 / 3.0 * math.pi * r ** 3, rel_tol=4e-16)
         assert m == float(mass_of([r])[0])
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fast_mode_equals_reference_loops(oracle, dtype):
+    """The oracle's fast mode (triangle-grid index for the mesh loop, threaded pair loop), used for
+    the long test runs, gives bit for bit the results of the reference's loops: a mesh scene with
+    wall and floor contacts, a hopper cone, and a dense box."""
+    from helpers import dense_box, small_cylinder
+    from psx.sim import Scene
+    cases = [(small_cylinder(dtype=dtype), 150)]
+    rng = np.random.default_rng(4)
+    hop = meshes.hopper(outlet_radius=3.0, upper_radius=12.0, half_angle_deg=35.0,
+                        upper_height=20.0, n_az=48, n_cone=6, n_cyl=4, n_floor=48)
+    hp = rng.uniform([-4, -4, 6], [4, 4, 14], size=(600, 3))
+    hr = rng.uniform(0.3, 0.6, 600)
+    cases.append((Scene(hp, rng.normal(0, 1, (600, 3)), hr,
+                        Params(gravity=(0, 0, -9.8), walls=(0.4,) * 6, spring=200.0,
+                               restitution=0.4, dt=1e-3, dtype=dtype), mass_of(hr), hop), 300))
+    cases.append((dense_box(n_side=8, dtype=dtype), 100))
+    for sc, steps in cases:
+        a = oracle.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps,
+                       mesh=sc.mesh, dtype=dtype)
+        b = oracle.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps,
+                       mesh=sc.mesh, dtype=dtype, fast=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert not np.array_equal(a[0], np.asarray(sc.pos, a[0].dtype))
