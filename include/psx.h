@@ -78,7 +78,9 @@ typedef struct psx_tuning {
                                (at most 512 for PSX_F64) */
   int32_t block_pairs;      /* workgroup size of the pair-contact kernel */
   int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
-  int32_t pair_variant;     /* lanes per particle in the pair-contact kernel: 0 or 1, 2, 4, 8, 16 */
+  int32_t pair_variant;     /* pair-contact kernel: lanes per particle 0 or 1, 2, 4, 8, 16; or 32: one
+                               lane per particle with the workgroup's neighbour particles staged in
+                               LDS (block_pairs up to 512, 256 in binary64) */
   int32_t sort_variant;     /* cell lists of the bucketing: 1 per-cell id blocks (four kernels per
                                step), 2 dense lists filled by a scatter kernel (five), 0 automatic
                                (id blocks up to 2^18 cells) */

@@ -1043,6 +1043,109 @@ __global__ void __launch_bounds__(1024)
   });
 }
 
+// The one-lane pair kernel with the workgroup's neighbour particles staged in LDS. A workgroup's
+// particles are consecutive in cell order, so their home cells lie in the range [ca, cb] of its
+// first and last particle's home cells (give or take a container push), and the members of every
+// neighbour column slice of a cell in that range lie in one contiguous run of the sorted arrays
+// per column offset (dx, dy): the run of cells [ca + off - 1, cb + off + 1]. The nine runs are
+// copied to LDS with coalesced loads (plus kPairPad spare entries each, for the four-wide reads);
+// each particle then reads its candidates from LDS when its slice lies inside the staged run (any
+// particle whose home left the range reads them from memory, as does a workgroup whose runs
+// exceed its LDS share). The candidates, their order and the arithmetic are those of k_pairs.
+template <class T, bool kTable>
+__global__ void __launch_bounds__(512)
+    k_pairs_staged(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
+                   const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
+                   const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout,
+                   const NextBucket next) {
+  extern __shared__ __align__(16) unsigned char stage_raw[];
+  Vec4<T>* const stage = reinterpret_cast<Vec4<T>*>(stage_raw);
+  __shared__ WgTiles wt;
+  __shared__ uint32_t s_lo[9], s_hi[9], s_base[9];
+  __shared__ uint32_t s_total;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    const bool tiles = next.count && next.tiles.tc;
+    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    if (threadIdx.x < 9) {
+      const int d = (int)threadIdx.x;
+      const GridDev& g = sc.grid;
+      const int32_t h0 = (int32_t)(home[p0] & ~kHomeCull), h1 = (int32_t)(home[p1 - 1] & ~kHomeCull);
+      const int32_t col = ((d / 3 - 1) * g.n[1] + (d % 3 - 1)) * g.n[2];
+      const int32_t first = g.cell_base, last = g.cell_base + g.n[0] * g.n[1] * g.n[2] - 1;
+      const int32_t cs = max(min(h0, h1) + col - 1, first), ce = min(max(h0, h1) + col + 1, last);
+      uint32_t lo = 0, hi = 0;
+      if (cs <= ce) {
+        lo = offset[cs];
+        hi = offset[ce + 1];
+      }
+      s_lo[d] = lo;
+      s_hi[d] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b = 0;
+      for (int d = 0; d < 9; ++d) {
+        s_base[d] = b;
+        b += (s_hi[d] - s_lo[d]) + (uint32_t)kPairPad;
+      }
+      s_total = b;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
+    const bool staged = total <= (uint32_t)(kStageFactor * (int)blockDim.x);
+    if (staged) {
+      // entry i of the staged runs belongs to run d with s_base[d] <= i < s_base[d + 1]; every
+      // thread loads its (at most kStageFactor) entries at once, then stores them
+      Vec4<T> buf[kStageFactor];
+#pragma unroll
+      for (int k = 0; k < kStageFactor; ++k) {
+        const uint32_t i = threadIdx.x + (uint32_t)k * blockDim.x;
+        if (i < total) {
+          int d = 0;
+#pragma unroll
+          for (int e = 1; e < 9; ++e) d += (i >= s_base[e]) ? 1 : 0;
+          // the sorted positions carry kPairPad spare entries, so the pads stay in the array
+          buf[k] = pos[s_lo[d] + (i - s_base[d])];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kStageFactor; ++k) {
+        const uint32_t i = threadIdx.x + (uint32_t)k * blockDim.x;
+        if (i < total) stage[i] = buf[k];
+      }
+    }
+    __syncthreads();
+    int32_t c = -1;
+    if (p < p1) {
+      PairSelf<T> me;
+      uint32_t lo[9], hi[9];
+      pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+      T ax = me.v.x, ay = me.v.y, az = me.v.z;
+#pragma unroll
+      for (int d = 0; d < 9; ++d) {
+        const uint32_t e = hi[d];
+        if (staged && lo[d] >= s_lo[d] && e <= s_hi[d]) {
+          const Vec4<T>* src = stage + s_base[d] - s_lo[d];
+          for (uint32_t q = lo[d]; q < e; q += 4)
+            pair_test4(me, src[q], src[q + 1], src[q + 2], src[q + 3], vsn, q, e, (uint32_t)p, ax,
+                       ay, az);
+        } else {
+          for (uint32_t q = lo[d]; q < e; q += 4)
+            pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
+        }
+      }
+      const Vec4<T> vn{ax, ay, az, me.v.w};
+      vout[p] = vn;
+      if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+    }
+    if (tiles) {
+      wt.add(next.tiles, c);
+      wt.end(next.tiles);
+    }
+  });
+}
+
 // ---------------------------------------------------------------------------------------------
 // Statistics of the current state (ParticleStatistics::calculateStatistics,
 // ParticleStatistics.cpp:3-43), on a bucketing of the current positions: kinetic energy
@@ -1309,6 +1412,11 @@ static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int bloc
     case 16:
       PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
+    case kPairStaged:
+      PSX_LAUNCH((k_pairs_staged<T, kTable>), grid, blk,
+                 (size_t)kStageFactor * (size_t)block * sizeof(Vec4<T>), st, loc, pos, vsn, offset,
+                 home, vout, next);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -1320,7 +1428,8 @@ hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const Bl
                         const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
                         const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
                         Vec4<T>* vout, const NextBucket& next) {
-  const Locator<T> loc = make_locator(ntab, block / lanes, btab, scenes, one);
+  const Locator<T> loc =
+      make_locator(ntab, lanes == kPairStaged ? block : block / lanes, btab, scenes, one);
   if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
   return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
 }

@@ -126,7 +126,13 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
                               int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o, uint32_t* gen);
 
-// lanes: lanes per particle (1, 2, 4, 8 or 16); a workgroup holds block / lanes particles.
+// The pair kernel form that stages the workgroup's neighbour particles in LDS (pair_variant value;
+// one lane per particle): at most kStageFactor sorted positions per workgroup thread.
+constexpr int kPairStaged = 32;
+constexpr int kStageFactor = 12;
+
+// lanes: lanes per particle (1, 2, 4, 8 or 16), or kPairStaged; a workgroup holds block / lanes
+// particles (block for kPairStaged).
 // home: the home word of every sorted slot (from the rank/gather pass that produced pos).
 template <class T>
 hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,

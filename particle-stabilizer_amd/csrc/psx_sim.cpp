@@ -612,9 +612,12 @@ struct SimImpl final : SimBase {
   // A single scene launches without the workgroup table (constants by value, slices by index).
   bool single() const { return scenes_h.size() == 1; }
 
-  // Lanes per particle of the pair kernel (tuning.pair_variant; 0 means 1).
+  // Form of the pair kernel (tuning.pair_variant; 0 means 1): lanes per particle, or kPairStaged
+  // (one lane per particle, neighbour particles staged in LDS).
   int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
-  int pair_particles_per_block() const { return tuning.block_pairs / pair_lanes(); }
+  int pair_particles_per_block() const {
+    return pair_lanes() == kPairStaged ? tuning.block_pairs : tuning.block_pairs / pair_lanes();
+  }
 
   // Arguments of a hash pass done inside another kernel, for the particles with ids `ids` (in the
   // kernel's particle order); ids == nullptr: none.
@@ -1339,8 +1342,15 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
   if (c.sort_variant < 0 || c.sort_variant > 2)
     return set_error(PSX_ERR_INVALID, "sort_variant must be 0 (automatic), 1 (id blocks) or 2 (dense lists)");
   if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
-      c.pair_variant != 8 && c.pair_variant != 16)
-    return set_error(PSX_ERR_INVALID, "pair_variant (lanes per particle) must be 0, 1, 2, 4, 8 or 16");
+      c.pair_variant != 8 && c.pair_variant != 16 && c.pair_variant != psx::kPairStaged)
+    return set_error(PSX_ERR_INVALID,
+                     "pair_variant must be 0, 1, 2, 4, 8 or 16 (lanes per particle) or 32 (LDS-staged)");
+  // the staged form takes kStageFactor positions per thread of LDS: at most 96 KiB
+  const int vec_bytes = sim->impl->dtype() == PSX_F64 ? 32 : 16;
+  if (c.pair_variant == psx::kPairStaged && psx::kStageFactor * c.block_pairs * vec_bytes > 96 * 1024)
+    return set_error(PSX_ERR_INVALID, vec_bytes == 32
+                                          ? "the LDS-staged pair kernel takes block_pairs up to 256 in binary64"
+                                          : "the LDS-staged pair kernel takes block_pairs up to 512");
   PSX_TRY({
     sim->impl->retune(c);
     return PSX_OK;

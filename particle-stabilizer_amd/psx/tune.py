@@ -7,9 +7,11 @@ each kernel keeps the shape with the smallest median time over its rounds. Tuned
 
 - block_integrate: workgroup size of the integrate / hash / container kernel;
 - block_sort: workgroup size of the rank / gather / container kernel (at most 512 in binary64);
-- (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel;
-- sort_variant: per-cell id blocks or dense lists for the bucketing (half of the rounds each; the
-  variant with the smaller hash + scan + scatter + gather time wins).
+- (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel, or its
+  LDS-staged form (pair_variant 32, workgroups of 64 to 256);
+- sort_variant: per-cell id slots or dense lists for the cell lists (half of the rounds each; the
+  form whose kernels take less time per step wins, the pair kernel included, since it does the
+  next step's hash pass).
 
 The cell size stays what the reference's grid rule fixes (it changes the neighbour search, not
 only the speed), and every launch shape gives bit-identical results, so tuning never changes a
@@ -25,7 +27,8 @@ from pathlib import Path
 import numpy as np
 
 BLOCKS = (64, 128, 256, 512, 1024)
-PAIR_SHAPES = tuple((b, g) for g in (1, 2, 4, 8, 16) for b in (64, 128, 256, 512, 1024))
+PAIR_SHAPES = (tuple((b, g) for g in (1, 2, 4, 8, 16) for b in (64, 128, 256, 512, 1024))
+               + tuple((b, 32) for b in (64, 128, 256)))
 
 
 def signature(sim) -> str:

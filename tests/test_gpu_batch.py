@@ -22,13 +22,15 @@ def _scenes(dtype):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("graph", [False, True])
-def test_batch_equals_plain_and_oracle(psx_mod, oracle, dtype, graph):
+@pytest.mark.parametrize("graph,staged", [(False, False), (True, False), (True, True)])
+def test_batch_equals_plain_and_oracle(psx_mod, oracle, dtype, graph, staged):
     scenes = _scenes(dtype)
     steps = 40
     with psx_mod.Batch(scenes) as b:
         if graph:
             b.set_tuning(use_graph=1)
+        if staged:  # the LDS-staged pair kernel on the scenes' cell segments
+            b.set_tuning(pair_variant=32, block_pairs=128)
         b.step(steps)
         b.synchronize()
         bp, bv = b.state()

@@ -91,6 +91,11 @@ def test_bucketing_cell_overflow(psx_mod, oracle, dtype):
         p, v = hip_run(sc, 3, sort_variant=variant)
         assert_bitwise(as_dtype(p, dtype), ref_p, f"pos sort_variant={variant}")
         assert_bitwise(as_dtype(v, dtype), ref_v, f"vel sort_variant={variant}")
+    # the LDS-staged pair kernel: the 300-sphere cell overfills a workgroup's LDS share, so some
+    # workgroups read from memory and the others from LDS
+    p, v = hip_run(sc, 3, pair_variant=32, block_pairs=64)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos staged")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel staged")
 
 
 VARIANTS = [dict(block_pairs=256, block_integrate=256),
@@ -98,7 +103,9 @@ VARIANTS = [dict(block_pairs=256, block_integrate=256),
             dict(block_pairs=64, block_integrate=1024, block_sort=64, pair_variant=4, sort_variant=1),
             dict(block_pairs=1024, block_integrate=128, block_sort=1024, pair_variant=8,
                  sort_variant=2),
-            dict(block_pairs=512, pair_variant=16, sort_variant=1)]
+            dict(block_pairs=512, pair_variant=16, sort_variant=1),
+            dict(block_pairs=256, block_sort=128, pair_variant=32, sort_variant=1),
+            dict(block_pairs=64, block_integrate=512, pair_variant=32, sort_variant=2)]
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -147,9 +154,10 @@ def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("lanes", [2, 4, 8, 16])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32])
 def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
-    """Multi-lane pair kernel on the mesh scene, eager and graph-replayed."""
+    """Multi-lane pair kernel (and the LDS-staged one, pair_variant 32) on the mesh scene, eager
+    and graph-replayed; container pushes move some homes out of their workgroup's staged range."""
     sc = small_cylinder(dtype=dtype)
     ref_p, ref_v = oracle_run(oracle, sc, 40)
     for graph in (False, True):
@@ -161,9 +169,11 @@ def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
 def test_tuning_rejects_bad_lanes(psx_mod):
     sc = dense_box(dtype="f64")
     with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
-        for bad in (3, 32, -1):
+        for bad in (3, 64, -1):
             with pytest.raises(psx_mod.PsxError):
                 s.set_tuning(pair_variant=bad)
+        with pytest.raises(psx_mod.PsxError, match="256"):
+            s.set_tuning(pair_variant=32, block_pairs=512)  # binary64: LDS share too large
         s.set_tuning(pair_variant=4)
         assert s.tuning["pair_variant"] == 4
         with pytest.raises(psx_mod.PsxError, match="512"):
