@@ -101,7 +101,7 @@ def bench_hopper(steps: int = 100, warmup: int = 10) -> dict:
     sim.set_state(p0, v0)
     prof = sim.profile(min(steps, 50))
     sv = tuning.get("sort_variant", 0)
-    algo = algorithmic_bytes(n, sim.num_cells, 4, sv == 1 or (sv == 0 and sim.num_cells <= 1 << 18))
+    algo = algorithmic_bytes(n, sim.num_cells, 4, sv != 2)
     kern = {k: {"ms": round(d["ms"], 4), "launches_per_step": round(d["per_step"], 4),
                 "GBps": round(algo[k] / (d["ms"] * 1e-3) / 1e9, 1) if algo.get(k) and d["ms"] else None}
             for k, d in prof.items() if d["per_step"] > 0}
@@ -271,7 +271,7 @@ def main() -> int:
     prof = sim.profile(psteps)
     cells = sim.num_cells
     sv = tuning.get("sort_variant", 0)
-    blocks = sv == 1 or (sv == 0 and cells <= 1 << 18)
+    blocks = sv != 2
     algo = algorithmic_bytes(n, cells, s_bytes, blocks)
     kernels = {}
     for name, d in prof.items():

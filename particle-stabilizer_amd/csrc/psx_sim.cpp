@@ -524,18 +524,14 @@ struct SimImpl final : SimBase {
   }
 
   // Bucketing: k_integrate hashes the positions of (ps, vs) after the integration step (when `mode`
-  // says so) and files the ids `is` into the cell id blocks; scan into `off`; k_rank_gather ranks,
+  // says so) and files the ids `is` into the cells' id slots; scan into `off`; k_rank_gather ranks,
   // integrates, applies the container (kModeContainer) and the check (kModeCheck) to (ps, vs) and
   // gathers into (pd, vd, idd).
-  // Cell lists of the bucketing: per-cell id blocks (one kernel fewer per step, but every id lands
-  // in its own sparse 32-byte block) or dense lists (a scatter kernel, contiguous lists). On MI355X
-  // the id blocks pay off only on small grids, where the launch they save is a visible share of
-  // the step (measured: equal at 4e5 cells, 25-30 % slower from 1e7 cells).
-  bool id_blocks() const {
-    if (tuning.sort_variant == 1) return true;
-    if (tuning.sort_variant == 2) return false;
-    return cpad <= (int64_t(1) << 18);
-  }
+  // Cell lists of the bucketing: per-cell id slots, slot-major (no scatter kernel; consecutive
+  // spheres fill neighbouring words) or dense lists (a scatter kernel, the hash pass keeps only the
+  // counts). Measured on MI355X, the id slots are faster on all four standard workloads (262k to
+  // 4.6M spheres, 4e5 to 3.5e7 cells), so they are the automatic choice.
+  bool id_blocks() const { return tuning.sort_variant != 2; }
 
   TileCounts tiles() const {
     TileCounts t;
