@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PSX_ABI_VERSION 2
+#define PSX_ABI_VERSION 3
 
 enum psx_status {
   PSX_OK = 0,
@@ -84,6 +84,8 @@ typedef struct psx_tuning {
   int32_t sort_variant;     /* cell lists of the bucketing: 1 per-cell id slots (three kernels per
                                step in a run: scan, rank/gather, pairs), 2 dense lists filled by a
                                scatter kernel (four), 0 automatic (id slots) */
+  int32_t pair_chunk;       /* candidates per round of the one-lane pair kernel (pair_variant 0 or 1):
+                               0 or 4, or 2 (fewer wasted tests where column slices hold few spheres) */
 } psx_tuning;
 
 typedef struct psx_sim psx_sim;

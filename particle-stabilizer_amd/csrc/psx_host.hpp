@@ -51,6 +51,7 @@ inline psx_tuning default_tuning() {
   t.use_graph = 0;
   t.pair_variant = 0;
   t.sort_variant = 0;
+  t.pair_chunk = 0;
   return t;
 }
 

@@ -908,54 +908,87 @@ __device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>*
   pair_test4(me, src[0], src[1], src[2], src[3], vsn, q, e, self, ax, ay, az);
 }
 
-// The contact sum of particle p over its nine column slices, in slice order. In binary32 the first
-// chunks of the next kPairAhead slices are loaded while the current slice is tested, so the slices'
-// loads overlap instead of forming a chain of nine round trips (binary64 lacks the registers).
-// Measured on MI355X: two or three slices ahead change the 262k-, 4.2M- and 4.6M-sphere workloads
-// by about 1 % either way (more registers, fewer waves per SIMD), so one is kept.
-constexpr int kPairAhead = 1;
-
+// Two candidates at a time (the chunk c0, c1 = candidates q, q + 1 of a slice ending at e): for
+// slices that hold one or two spheres, half the prefilters of a four-wide chunk.
 template <class T>
+__device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>& c0,
+                                           const Vec4<T>& c1, const Vec4<T>* __restrict__ vsn,
+                                           uint32_t q, uint32_t e, uint32_t self, T& ax, T& ay,
+                                           T& az) {
+  const bool m0 = q < e && q != self && pair_near(me, c0);
+  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
+  if (m0 || m1) {
+    T fx, fy, fz;
+    if (m0 && pair_exact(me, c0, vsn, q, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+    if (m1 && pair_exact(me, c1, vsn, q + 1, fx, fy, fz)) {
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+  }
+}
+
+// The contact sum of particle p over its nine column slices, in slice order, W (4 or 2) candidates
+// per round. In binary32 the first chunk of the next slice is loaded while the current slice is
+// tested, so the slices' loads overlap instead of forming a chain of nine round trips (binary64
+// lacks the registers). Measured on MI355X: prefetching two or three slices ahead changes the
+// 262k-, 4.2M- and 4.6M-sphere workloads by about 1 % either way (more registers, fewer waves per
+// SIMD), so one is kept; two-wide chunks are 11 % faster on the 4.2M-sphere hopper and 5 % slower
+// on the 262k-sphere cylinder, so the chunk width is a tuning choice (psx_tuning.pair_chunk).
+template <class T, int W>
 __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
                                          const Vec4<T>* __restrict__ vsn,
                                          const uint32_t (&lo)[9], const uint32_t (&hi)[9],
                                          uint32_t self, T& ax, T& ay, T& az) {
+  static_assert(W == 2 || W == 4, "two or four candidates per round");
   if constexpr (sizeof(T) == 4) {
-    constexpr int D = kPairAhead;
-    Vec4<T> buf[D][4];
+    Vec4<T> buf[W];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const Vec4<T>* __restrict__ src = pos + lo[k];
-      buf[k][0] = src[0];
-      buf[k][1] = src[1];
-      buf[k][2] = src[2];
-      buf[k][3] = src[3];
-    }
+    for (int k = 0; k < W; ++k) buf[k] = pos[lo[0] + k];
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-      const Vec4<T> c0 = buf[d % D][0], c1 = buf[d % D][1], c2 = buf[d % D][2], c3 = buf[d % D][3];
-      if (d + D < 9) {
-        const Vec4<T>* __restrict__ nx = pos + lo[d + D];
-        buf[d % D][0] = nx[0];
-        buf[d % D][1] = nx[1];
-        buf[d % D][2] = nx[2];
-        buf[d % D][3] = nx[3];
+      Vec4<T> c[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) c[k] = buf[k];
+      if (d + 1 < 9) {
+        const Vec4<T>* __restrict__ nx = pos + lo[d + 1];
+#pragma unroll
+        for (int k = 0; k < W; ++k) buf[k] = nx[k];
       }
       const uint32_t e = hi[d];
       // unconditional (the masks cover an empty slice), so the prefetched chunk is used as loaded
-      pair_test4(me, c0, c1, c2, c3, vsn, lo[d], e, self, ax, ay, az);
-      for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+      if constexpr (W == 4) {
+        pair_test4(me, c[0], c[1], c[2], c[3], vsn, lo[d], e, self, ax, ay, az);
+        for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+      } else {
+        pair_test2(me, c[0], c[1], vsn, lo[d], e, self, ax, ay, az);
+        for (uint32_t q = lo[d] + 2; q < e; q += 2) {
+          const Vec4<T>* __restrict__ src = pos + q;
+          pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+        }
+      }
     }
   } else {
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
       const uint32_t e = hi[d];
-      for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+      if constexpr (W == 4) {
+        for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+      } else {
+        for (uint32_t q = lo[d]; q < e; q += 2) {
+          const Vec4<T>* __restrict__ src = pos + q;
+          pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+        }
+      }
     }
   }
 }
 
-template <class T, bool kTable>
+template <class T, bool kTable, int W>
 __global__ void __launch_bounds__(1024)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
@@ -971,7 +1004,7 @@ __global__ void __launch_bounds__(1024)
       uint32_t lo[9], hi[9];
       pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
       T ax = me.v.x, ay = me.v.y, az = me.v.z;
-      pair_sum(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
+      pair_sum<T, W>(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
       const Vec4<T> vn{ax, ay, az, me.v.w};
       vout[p] = vn;
       if (next.count) c = next_bucket(sc, next, me.x, vn, p);
@@ -1393,12 +1426,16 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
 
 template <class T, bool kTable>
 static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int block, int lanes,
-                                 const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
-                                 const uint32_t* home, Vec4<T>* vout, const NextBucket& next) {
+                                 int chunk, const Vec4<T>* pos, const Vec4<T>* vsn,
+                                 const uint32_t* offset, const uint32_t* home, Vec4<T>* vout,
+                                 const NextBucket& next) {
   const dim3 grid(xcd_grid(loc.ntab)), blk(block);
   switch (lanes) {
     case 1:
-      PSX_LAUNCH((k_pairs<T, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
+      if (chunk == 2)
+        PSX_LAUNCH((k_pairs<T, kTable, 2>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
+      else
+        PSX_LAUNCH((k_pairs<T, kTable, 4>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
     case 2:
       PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
@@ -1424,14 +1461,15 @@ static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int bloc
 }
 
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
-                        const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                        const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
-                        Vec4<T>* vout, const NextBucket& next) {
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, int chunk,
+                        const BlockInfo* btab, const SceneDev<T>* scenes, const SceneDev<T>* one,
+                        const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
+                        const uint32_t* home, Vec4<T>* vout, const NextBucket& next) {
   const Locator<T> loc =
       make_locator(ntab, lanes == kPairStaged ? block : block / lanes, btab, scenes, one);
-  if (one) return launch_pairs_t<T, false>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
-  return launch_pairs_t<T, true>(st, loc, block, lanes, pos, vsn, offset, home, vout, next);
+  if (one)
+    return launch_pairs_t<T, false>(st, loc, block, lanes, chunk, pos, vsn, offset, home, vout, next);
+  return launch_pairs_t<T, true>(st, loc, block, lanes, chunk, pos, vsn, offset, home, vout, next);
 }
 
 template <class T>
@@ -1480,7 +1518,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                             const int32_t*, uint32_t*, int, const uint8_t*,    \
                                             Vec4<T>*, Vec4<T>*, int32_t*, int32_t*,            \
                                             uint32_t*, Vec4<T>*, uint32_t*);                              \
-  template hipError_t launch_pairs<T>(hipStream_t, int, int, int, const BlockInfo*,            \
+  template hipError_t launch_pairs<T>(hipStream_t, int, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \
                                       Vec4<T>*, const NextBucket&);                            \

@@ -132,13 +132,13 @@ constexpr int kPairStaged = 32;
 constexpr int kStageFactor = 12;
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16), or kPairStaged; a workgroup holds block / lanes
-// particles (block for kPairStaged).
+// particles (block for kPairStaged). chunk: candidates per round of the one-lane kernel (4 or 2).
 // home: the home word of every sorted slot (from the rank/gather pass that produced pos).
 template <class T>
-hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, const BlockInfo* btab,
-                        const SceneDev<T>* scenes, const SceneDev<T>* one, const Vec4<T>* pos,
-                        const Vec4<T>* vsn, const uint32_t* offset, const uint32_t* home,
-                        Vec4<T>* vout, const NextBucket& next);
+hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, int chunk,
+                        const BlockInfo* btab, const SceneDev<T>* scenes, const SceneDev<T>* one,
+                        const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
+                        const uint32_t* home, Vec4<T>* vout, const NextBucket& next);
 
 // Per-workgroup partial statistics (combined on the host in workgroup order).
 struct StatPart {

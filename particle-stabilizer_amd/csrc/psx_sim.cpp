@@ -643,7 +643,7 @@ struct SimImpl final : SimBase {
     const NextBucket nb = next_bucket_args(next_ids);
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
-                             single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
+                             tuning.pair_chunk == 2 ? 2 : 4, single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
                              single() ? scenes_h.data() : nullptr, ps, vs, off,
                              home.as<uint32_t>(), vout, nb);
     });
@@ -1341,6 +1341,8 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
       c.pair_variant != 8 && c.pair_variant != 16 && c.pair_variant != psx::kPairStaged)
     return set_error(PSX_ERR_INVALID,
                      "pair_variant must be 0, 1, 2, 4, 8 or 16 (lanes per particle) or 32 (LDS-staged)");
+  if (c.pair_chunk != 0 && c.pair_chunk != 2 && c.pair_chunk != 4)
+    return set_error(PSX_ERR_INVALID, "pair_chunk must be 0 or 4, or 2");
   // the staged form takes kStageFactor positions per thread of LDS: at most 96 KiB
   const int vec_bytes = sim->impl->dtype() == PSX_F64 ? 32 : 16;
   if (c.pair_variant == psx::kPairStaged && psx::kStageFactor * c.block_pairs * vec_bytes > 96 * 1024)

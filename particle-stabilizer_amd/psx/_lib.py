@@ -50,7 +50,8 @@ class psx_stats(C.Structure):
 class psx_tuning(C.Structure):
     _fields_ = [("block_integrate", C.c_int32), ("block_sort", C.c_int32),
                 ("block_pairs", C.c_int32), ("use_graph", C.c_int32),
-                ("pair_variant", C.c_int32), ("sort_variant", C.c_int32)]
+                ("pair_variant", C.c_int32), ("sort_variant", C.c_int32),
+                ("pair_chunk", C.c_int32)]
 
 
 _DP = C.POINTER(C.c_double)

@@ -7,8 +7,9 @@ each kernel keeps the shape with the smallest median time over its rounds. Tuned
 
 - block_integrate: workgroup size of the integrate / hash / container kernel;
 - block_sort: workgroup size of the rank / gather / container kernel (at most 512 in binary64);
-- (block_pairs, pair_variant): workgroup size and lanes per particle of the pair kernel, or its
-  LDS-staged form (pair_variant 32, workgroups of 64 to 256);
+- (block_pairs, pair_variant, pair_chunk): workgroup size and lanes per particle of the pair
+  kernel, or its LDS-staged form (pair_variant 32, workgroups of 64 to 256), and the one-lane
+  kernel's candidates per round (4 or 2);
 - sort_variant: per-cell id slots or dense lists for the cell lists (half of the rounds each; the
   form whose kernels take less time per step wins, the pair kernel included, since it does the
   next step's hash pass).
@@ -27,8 +28,9 @@ from pathlib import Path
 import numpy as np
 
 BLOCKS = (64, 128, 256, 512, 1024)
-PAIR_SHAPES = (tuple((b, g) for g in (1, 2, 4, 8, 16) for b in (64, 128, 256, 512, 1024))
-               + tuple((b, 32) for b in (64, 128, 256)))
+PAIR_SHAPES = (tuple((b, g, 4) for g in (1, 2, 4, 8, 16) for b in (64, 128, 256, 512, 1024))
+               + tuple((b, 1, 2) for b in (64, 128, 256, 512, 1024))
+               + tuple((b, 32, 4) for b in (64, 128, 256)))
 
 
 def signature(sim) -> str:
@@ -71,15 +73,15 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
         sv = 1 if k < n_half else 2
         bi = BLOCKS[k % len(BLOCKS)]
         bs = sort_blocks[(k // len(BLOCKS)) % len(sort_blocks)]
-        bp, g = PAIR_SHAPES[k % len(PAIR_SHAPES)]
+        bp, g, w = PAIR_SHAPES[k % len(PAIR_SHAPES)]
         sim.set_tuning(block_integrate=bi, block_sort=bs, block_pairs=bp, pair_variant=g,
-                       sort_variant=sv, use_graph=0)
+                       pair_chunk=w, sort_variant=sv, use_graph=0)
         prof = sim.profile(steps)
         for name, d in prof.items():
             # the pair kernel also does the next step's hash pass, whose cost depends on the
             # cell-list form: its samples are kept per form
             shape = {"k_integrate": (bi, sv), "k_scan": (0, sv), "k_rank_gather": (bs, sv),
-                     "k_scatter_ids": (bs, sv), "k_pairs": (bp, g, sv)}.get(name)
+                     "k_scatter_ids": (bs, sv), "k_pairs": (bp, g, w, sv)}.get(name)
             per = d.get("per_step", 1)
             if shape is None or per == 0:
                 continue
@@ -109,20 +111,21 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
             sc = med("k_scatter_ids", bs, sv) if sv == 2 else 0.0
             if r is not None and sc is not None:
                 bs_ms[bs] = r * rate("k_rank_gather", sv) + sc * rate("k_scatter_ids", sv)
-        pair_sv = {(bp, g): med("k_pairs", bp, g, sv) for bp, g in PAIR_SHAPES}
+        pair_sv = {(bp, g, w): med("k_pairs", bp, g, w, sv) for bp, g, w in PAIR_SHAPES}
         pair_sv = {k: t for k, t in pair_sv.items() if t is not None}
         scan = (med("k_scan", 0, sv) or 0.0) * rate("k_scan", sv)
         if bi_ms and bs_ms and pair_sv:
             bi = min(bi_ms, key=bi_ms.get)
             bs = min(bs_ms, key=bs_ms.get)
-            bp, g = min(pair_sv, key=pair_sv.get)
+            bp, g, w = min(pair_sv, key=pair_sv.get)
             total = (bi_ms[bi] * rate("k_integrate", sv) + bs_ms[bs] + scan
-                     + pair_sv[(bp, g)] * rate("k_pairs", sv))
-            plans[sv] = (total, bi, bs, bp, g, pair_sv[(bp, g)])
+                     + pair_sv[(bp, g, w)] * rate("k_pairs", sv))
+            plans[sv] = (total, bi, bs, bp, g, w, pair_sv[(bp, g, w)])
     sv_best = min(plans, key=lambda v: plans[v][0])
-    _, bi_best, bs_best, bp_best, g_best, pair_best = plans[sv_best]
+    _, bi_best, bs_best, bp_best, g_best, w_best, pair_best = plans[sv_best]
     tuning = {"block_integrate": int(bi_best), "block_sort": int(bs_best),
               "block_pairs": int(bp_best), "pair_variant": int(g_best),
+              "pair_chunk": int(w_best),
               "sort_variant": int(sv_best), "use_graph": 1 if use_graph else 0}
     sim.set_tuning(**tuning)
     sim.set_state(p0, v0)

@@ -105,6 +105,8 @@ VARIANTS = [dict(block_pairs=256, block_integrate=256),
                  sort_variant=2),
             dict(block_pairs=512, pair_variant=16, sort_variant=1),
             dict(block_pairs=256, block_sort=128, pair_variant=32, sort_variant=1),
+            dict(block_pairs=128, pair_variant=1, pair_chunk=2, sort_variant=1),
+            dict(block_pairs=1024, pair_chunk=2, sort_variant=2),
             dict(block_pairs=64, block_integrate=512, pair_variant=32, sort_variant=2)]
 
 
@@ -154,14 +156,16 @@ def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, "chunk2"])
 def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
-    """Multi-lane pair kernel (and the LDS-staged one, pair_variant 32) on the mesh scene, eager
-    and graph-replayed; container pushes move some homes out of their workgroup's staged range."""
+    """Multi-lane pair kernel (and the LDS-staged one, pair_variant 32, and the one-lane one with
+    two candidates per round) on the mesh scene, eager and graph-replayed; container pushes move
+    some homes out of their workgroup's staged range."""
     sc = small_cylinder(dtype=dtype)
     ref_p, ref_v = oracle_run(oracle, sc, 40)
+    shape = dict(pair_variant=1, pair_chunk=2) if lanes == "chunk2" else dict(pair_variant=lanes)
     for graph in (False, True):
-        p, v = hip_run(sc, 40, graph=graph, pair_variant=lanes, block_pairs=256)
+        p, v = hip_run(sc, 40, graph=graph, block_pairs=256, **shape)
         assert_bitwise(as_dtype(p, dtype), ref_p, f"pos lanes={lanes} graph={graph}")
         assert_bitwise(as_dtype(v, dtype), ref_v, f"vel lanes={lanes} graph={graph}")
 
@@ -172,6 +176,9 @@ def test_tuning_rejects_bad_lanes(psx_mod):
         for bad in (3, 64, -1):
             with pytest.raises(psx_mod.PsxError):
                 s.set_tuning(pair_variant=bad)
+        for bad in (1, 3, 8):
+            with pytest.raises(psx_mod.PsxError, match="pair_chunk"):
+                s.set_tuning(pair_chunk=bad)
         with pytest.raises(psx_mod.PsxError, match="256"):
             s.set_tuning(pair_variant=32, block_pairs=512)  # binary64: LDS share too large
         s.set_tuning(pair_variant=4)

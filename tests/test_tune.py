@@ -17,7 +17,7 @@ class FakeSim:
         self.num_cells = 4096
         self.num_scenes = 1
         self.tuning = {"block_integrate": 256, "block_sort": 256, "block_pairs": 256,
-                       "use_graph": 0, "pair_variant": 0, "sort_variant": 0}
+                       "use_graph": 0, "pair_variant": 0, "sort_variant": 0, "pair_chunk": 0}
         self.pos = np.arange(3000.0).reshape(1000, 3)
         self.vel = -self.pos
         self.profiled = 0
@@ -42,8 +42,8 @@ class FakeSim:
                             "per_step": 1},
             "k_scan": {"ms": 0.004, "per_step": 1},
             "k_rank_gather": {"ms": 0.005 + abs(t["block_sort"] - 128) * 1e-6, "per_step": 1},
-            "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6,
-                        "per_step": 1},
+            "k_pairs": {"ms": 0.02 + abs(g - 4) * 1e-3 + abs(t["block_pairs"] - 256) * 1e-6
+                        + (0.0 if t["pair_chunk"] in (0, 4) else 2e-4), "per_step": 1},
             "k_scatter_ids": {"ms": 0.003 if dense else 0.0, "per_step": 1 if dense else 0},
         }
 
@@ -53,7 +53,7 @@ def test_picks_fastest_shapes_and_restores_state():
     out = tune.autotune(sim, steps=2)
     # the hash kernel only starts a run: dense lists cost the scatter kernel's 0.003 ms per step
     assert out["tuning"] == {"block_integrate": 512, "block_sort": 128, "block_pairs": 256,
-                             "pair_variant": 4, "sort_variant": 1, "use_graph": 1}
+                             "pair_variant": 4, "pair_chunk": 4, "sort_variant": 1, "use_graph": 1}
     assert sim.tuning["pair_variant"] == 4 and sim.tuning["use_graph"] == 1
     assert sim.restored is not None
     np.testing.assert_array_equal(sim.restored[0], sim.pos)
@@ -90,6 +90,19 @@ def test_dense_lists_win_on_a_cheaper_pair_kernel():
             return out
 
     assert tune.autotune(DenseSim(), steps=4)["tuning"]["sort_variant"] == 2
+
+
+def test_two_wide_chunks_chosen_when_faster():
+    class ChunkSim(FakeSim):
+        def profile(self, steps):
+            out = super().profile(steps)
+            t = self.tuning
+            if t["pair_variant"] in (0, 1) and t["pair_chunk"] == 2:
+                out["k_pairs"]["ms"] = 0.001  # the one-lane, two-wide kernel wins
+            return out
+
+    tuning = tune.autotune(ChunkSim(), steps=2)["tuning"]
+    assert tuning["pair_variant"] == 1 and tuning["pair_chunk"] == 2
 
 
 def test_cache_written_and_reused(tmp_path):
