@@ -735,37 +735,50 @@ struct PairSelf {
 
 // The prefilter of candidate j (fused multiply-adds, conservative by the margin): false only for
 // candidates the exact test would reject. NaN coordinates pass it and fail the exact test.
-template <class T>
-__device__ __forceinline__ bool pair_near(const PairSelf<T>& me, const Vec4<T>& xj) {
+template <class T, class P>
+__device__ __forceinline__ bool pair_near(const PairSelf<T>& me, const P& xj) {
   const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
   return !(fma(ex, ex, fma(ey, ey, ez * ez)) > me.thr);
 }
 
-// The exact contact test and kick of candidate j on particle i, in the reference's expression
-// order; false when they do not touch.
+// The contact geometry of candidate j with particle i: separation, squared distance, radius sum
+// and distance.
 template <class T>
-__device__ __forceinline__ bool pair_exact(const PairSelf<T>& me, const Vec4<T>& xj,
-                                           const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
-                                           T& fy, T& fz) {
-  // isHit: distance(x_i, x_j) = length(x_j - x_i) < r_i + r_j
-  const T ex = xj.x - me.x.x, ey = xj.y - me.x.y, ez = xj.z - me.x.z;
-  const T dd = dot3(ex, ey, ez, ex, ey, ez);
-  const T R = me.x.w + xj.w;
-  if (dd > (R * R) * PairMargin<T>::value) return false;
-  const T len = sqrt(dd);
-  if (!(len < R)) return false;
-  const Vec4<T> vj = vsn[q];
+struct PairGeo {
+  T ex, ey, ez, dd, R, len;
+};
+
+// The exact contact test of candidate j on particle i (isHit: distance(x_i, x_j) = length(x_j -
+// x_i) < r_i + r_j); false when they do not touch. The squared-distance pretest is conservative by
+// the margin, so only non-contacts skip the square root.
+template <class T, class P>
+__device__ __forceinline__ bool pair_touch(const PairSelf<T>& me, const P& xj, T rj,
+                                           PairGeo<T>& g) {
+  g.ex = xj.x - me.x.x;
+  g.ey = xj.y - me.x.y;
+  g.ez = xj.z - me.x.z;
+  g.dd = dot3(g.ex, g.ey, g.ez, g.ex, g.ey, g.ez);
+  g.R = me.x.w + rj;
+  if (g.dd > (g.R * g.R) * PairMargin<T>::value) return false;
+  g.len = sqrt(g.dd);
+  return g.len < g.R;
+}
+
+// The kick of a touching candidate j (velocity vj) on particle i, in the reference's expression
+// order.
+template <class T>
+__device__ __forceinline__ void pair_force(const PairSelf<T>& me, const PairGeo<T>& g,
+                                           const Vec4<T>& vj, T& fx, T& fy, T& fz) {
   // d = x_i - x_j; dot(d, d) and length(d) equal dd and len bit for bit.
-  const T dx = -ex, dy = -ey, dz = -ez;
-  const T ov = R - len;
+  const T dx = -g.ex, dy = -g.ey, dz = -g.ez;
+  const T ov = g.R - g.len;
   const T s3 = ((T(2) * vj.w) / (me.v.w + vj.w)) *
-               (dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz) / dd);
-  const T inv = T(1) / len;
+               (dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz) / g.dd);
+  const T inv = T(1) / g.len;
   const T kov = me.ks * ov;
   fx = (-((s3 * dx) * me.es)) + kov * (dx * inv);
   fy = (-((s3 * dy) * me.es)) + kov * (dy * inv);
   fz = (-((s3 * dz) * me.es)) + kov * (dz * inv);
-  return true;
 }
 
 // One candidate j of particle i: its kick on i, or false when they do not touch.
@@ -773,19 +786,10 @@ template <class T>
 __device__ __forceinline__ bool pair_kick(const PairSelf<T>& me, const Vec4<T>& xj,
                                           const Vec4<T>* __restrict__ vsn, uint32_t q, T& fx,
                                           T& fy, T& fz) {
-  return pair_near(me, xj) && pair_exact(me, xj, vsn, q, fx, fy, fz);
-}
-
-template <class T>
-__device__ __forceinline__ void pair_candidate(const PairSelf<T>& me, const Vec4<T>& xj,
-                                               const Vec4<T>* __restrict__ vsn, uint32_t q,
-                                               T& ax, T& ay, T& az) {
-  T fx, fy, fz;
-  if (pair_kick(me, xj, vsn, q, fx, fy, fz)) {
-    ax = ax + fx;
-    ay = ay + fy;
-    az = az + fz;
-  }
+  PairGeo<T> g;
+  if (!(pair_near(me, xj) && pair_touch(me, xj, xj.w, g))) return false;
+  pair_force(me, g, vsn[q], fx, fy, fz);
+  return true;
 }
 
 // Particle i's constants and the nine column slices [lo[d], hi[d]) of its 27 neighbour cells.
@@ -859,57 +863,96 @@ __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
   }
 }
 
-// Candidates are taken four at a time: one address, four independent loads, four prefilters and a
-// single branch into the rare path, which applies the contacts of the chunk in ascending order.
-// The loads may run up to three entries past the end of a slice (or read four entries of an empty
-// slice at offset 0); the sorted position arrays carry kPairPad spare entries so that they stay
-// inside the allocation, and those entries are masked out.
-// Tests the chunk c0..c3 = candidates q .. q+3 of a slice ending at e.
+// A candidate that passed the prefilter: its contact test and, if they touch, its kick on
+// particle i, added to the sum; the candidate's velocity is loaded only for a contact.
 template <class T>
-__device__ __forceinline__ void pair_test4(const PairSelf<T>& me, const Vec4<T>& c0,
-                                           const Vec4<T>& c1, const Vec4<T>& c2,
-                                           const Vec4<T>& c3, const Vec4<T>* __restrict__ vsn,
-                                           uint32_t q, uint32_t e, uint32_t self, T& ax, T& ay,
-                                           T& az) {
-  const bool m0 = q < e && q != self && pair_near(me, c0);
-  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
-  const bool m2 = q + 2 < e && q + 2 != self && pair_near(me, c2);
-  const bool m3 = q + 3 < e && q + 3 != self && pair_near(me, c3);
-  if (m0 || m1 || m2 || m3) {
+__device__ __forceinline__ void pair_contact(const PairSelf<T>& me, const Vec4<T>& xj,
+                                             const Vec4<T>* __restrict__ vsn, uint32_t q, T& ax,
+                                             T& ay, T& az) {
+  PairGeo<T> g;
+  if (pair_touch(me, xj, xj.w, g)) {
     T fx, fy, fz;
-    if (m0 && pair_exact(me, c0, vsn, q, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
-    if (m1 && pair_exact(me, c1, vsn, q + 1, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
-    if (m2 && pair_exact(me, c2, vsn, q + 2, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
-    if (m3 && pair_exact(me, c3, vsn, q + 3, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
+    pair_force(me, g, vsn[q], fx, fy, fz);
+    ax = ax + fx;
+    ay = ay + fy;
+    az = az + fz;
   }
 }
 
+// Chunks of four candidates: one address, four independent loads, four prefilters and a single
+// branch into the rare path, which applies the contacts of the chunk in ascending order. The
+// masks are combined with bitwise operators, so every prefilter runs and the compiler issues the
+// four loads before the first test (one round trip per chunk; with short-circuit masks it sinks
+// each load under the previous candidate's condition: four round trips). The rare path loads the
+// four candidates' velocities in one go and waits for them at its end, so no load is in flight past
+// it (its registers would otherwise have to be waited for before the next slice's prefetch could
+// be issued). The loads may run up to three entries past the end of a slice (or read four entries
+// of an empty slice at offset 0); the sorted position and velocity arrays carry kPairPad spare
+// entries so that they stay inside the allocations, and those entries are masked out.
 template <class T>
-__device__ __forceinline__ void pair_chunk(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
-                                           const Vec4<T>* __restrict__ vsn, uint32_t q, uint32_t e,
-                                           uint32_t self, T& ax, T& ay, T& az) {
-  const Vec4<T>* __restrict__ src = pos + q;  // one address; the other three are offsets
-  pair_test4(me, src[0], src[1], src[2], src[3], vsn, q, e, self, ax, ay, az);
+__device__ __forceinline__ void pair_load4(const Vec4<T>* __restrict__ src, Vec4<T> (&c)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = src[k];
 }
 
-// Two candidates at a time (the chunk c0, c1 = candidates q, q + 1 of a slice ending at e): for
-// slices that hold one or two spheres, half the prefilters of a four-wide chunk.
+// vmcnt(0) with the other counters left alone (expcnt 7, lgkmcnt 15): every vector memory load of
+// the wave has returned.
+constexpr int kWaitVmem = 0xF70;
+
+// Tests the chunk c[0 .. 4) = candidates q .. q + 3 of a slice ending at e.
+template <class T>
+__device__ __forceinline__ void pair_test4(const PairSelf<T>& me, const Vec4<T> (&c)[4],
+                                           const Vec4<T>* __restrict__ vsn, uint32_t q, uint32_t e,
+                                           uint32_t self, T& ax, T& ay, T& az) {
+  bool m[4];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t j = q + (uint32_t)k;
+    m[k] = (j < e) & (j != self) & pair_near(me, c[k]);
+    any = any | m[k];
+  }
+  if (any) {
+    Vec4<T> vj[4];
+    pair_load4(vsn + q, vj);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      PairGeo<T> g;
+      if (m[k] && pair_touch(me, c[k], c[k].w, g)) {
+        T fx, fy, fz;
+        pair_force(me, g, vj[k], fx, fy, fz);
+        ax = ax + fx;
+        ay = ay + fy;
+        az = az + fz;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVmem);
+  }
+}
+
+// The chunk of four candidates starting at q inside a slice, read from memory: only the
+// candidates before the slice end e are loaded (the others are zero and masked out), so lanes past
+// their slice's end fetch nothing, and the loads are still issued together before the first test.
+template <class T>
+__device__ __forceinline__ void pair_chunk4(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                            const Vec4<T>* __restrict__ vsn, uint32_t q,
+                                            uint32_t e, uint32_t self, T& ax, T& ay, T& az) {
+  Vec4<T> c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c[k] = Vec4<T>{T(0), T(0), T(0), T(0)};
+    if (q + (uint32_t)k < e) c[k] = pos[q + k];
+  }
+  pair_test4(me, c, vsn, q, e, self, ax, ay, az);
+}
+
+// Chunks of two candidates (c0, c1 = candidates q, q + 1 of a slice ending at e), for slices that
+// hold one or two spheres: half the prefilters of a four-wide chunk. The masks short-circuit, so
+// a candidate past its slice's end costs its lane no load and a wave whose lanes are all past it
+// skips the test; the compiler then loads c1 under c0's condition (two round trips for a chunk
+// read inside a slice). Measured on MI355X, this beats the eager form of the four-wide chunks on
+// the workloads that fill the GPU (the 4.2M-sphere hopper, the 256-scene batch), whose pair
+// kernels are bound by instruction issue and load bytes rather than by latency.
 template <class T>
 __device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>& c0,
                                            const Vec4<T>& c1, const Vec4<T>* __restrict__ vsn,
@@ -918,17 +961,8 @@ __device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>&
   const bool m0 = q < e && q != self && pair_near(me, c0);
   const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
   if (m0 || m1) {
-    T fx, fy, fz;
-    if (m0 && pair_exact(me, c0, vsn, q, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
-    if (m1 && pair_exact(me, c1, vsn, q + 1, fx, fy, fz)) {
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
+    if (m0) pair_contact(me, c0, vsn, q, ax, ay, az);
+    if (m1) pair_contact(me, c1, vsn, q + 1, ax, ay, az);
   }
 }
 
@@ -937,8 +971,8 @@ __device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>&
 // tested, so the slices' loads overlap instead of forming a chain of nine round trips (binary64
 // lacks the registers). Measured on MI355X: prefetching two or three slices ahead changes the
 // 262k-, 4.2M- and 4.6M-sphere workloads by about 1 % either way (more registers, fewer waves per
-// SIMD), so one is kept; two-wide chunks are 11 % faster on the 4.2M-sphere hopper and 5 % slower
-// on the 262k-sphere cylinder, so the chunk width is a tuning choice (psx_tuning.pair_chunk).
+// SIMD), so one is kept; two-wide chunks are 7-11 % faster on the 4.2M-sphere hopper and 5 %
+// slower on the 262k-sphere cylinder, so the chunk width is a tuning choice (psx_tuning.pair_chunk).
 template <class T, int W>
 __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
                                          const Vec4<T>* __restrict__ vsn,
@@ -962,8 +996,8 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
       const uint32_t e = hi[d];
       // unconditional (the masks cover an empty slice), so the prefetched chunk is used as loaded
       if constexpr (W == 4) {
-        pair_test4(me, c[0], c[1], c[2], c[3], vsn, lo[d], e, self, ax, ay, az);
-        for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+        pair_test4(me, c, vsn, lo[d], e, self, ax, ay, az);
+        for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk4(me, pos, vsn, q, e, self, ax, ay, az);
       } else {
         pair_test2(me, c[0], c[1], vsn, lo[d], e, self, ax, ay, az);
         for (uint32_t q = lo[d] + 2; q < e; q += 2) {
@@ -977,7 +1011,7 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
     for (int d = 0; d < 9; ++d) {
       const uint32_t e = hi[d];
       if constexpr (W == 4) {
-        for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk(me, pos, vsn, q, e, self, ax, ay, az);
+        for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk4(me, pos, vsn, q, e, self, ax, ay, az);
       } else {
         for (uint32_t q = lo[d]; q < e; q += 2) {
           const Vec4<T>* __restrict__ src = pos + q;
@@ -1160,12 +1194,13 @@ __global__ void __launch_bounds__(512)
         const uint32_t e = hi[d];
         if (staged && lo[d] >= s_lo[d] && e <= s_hi[d]) {
           const Vec4<T>* src = stage + s_base[d] - s_lo[d];
-          for (uint32_t q = lo[d]; q < e; q += 4)
-            pair_test4(me, src[q], src[q + 1], src[q + 2], src[q + 3], vsn, q, e, (uint32_t)p, ax,
-                       ay, az);
+          for (uint32_t q = lo[d]; q < e; q += 4) {
+            Vec4<T> c[4];
+            pair_load4(src + q, c);
+            pair_test4(me, c, vsn, q, e, (uint32_t)p, ax, ay, az);
+          }
         } else {
-          for (uint32_t q = lo[d]; q < e; q += 4)
-            pair_chunk(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
+          for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk4(me, pos, vsn, q, e, (uint32_t)p, ax, ay, az);
         }
       }
       const Vec4<T> vn{ax, ay, az, me.v.w};

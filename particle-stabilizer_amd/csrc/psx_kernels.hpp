@@ -28,8 +28,8 @@ constexpr int kTriPad = 4;
 // Spare entries after the dense cell lists (the rank kernel reads a cell's first four at once).
 constexpr int kListPad = 4;
 
-// Spare entries after every sorted position array the pair kernel reads (its chunked loads may
-// reach three entries past the last particle).
+// Spare entries after every sorted position and velocity array the pair kernel reads (its chunked
+// loads may reach three entries past the last particle).
 constexpr int kPairPad = 4;
 
 // The next step's hash pass (launch_integrate's work on a cell-sorted state), done by the pair kernel

@@ -457,7 +457,7 @@ struct SimImpl final : SimBase {
     // device allocations
     for (int k = 0; k < 2; ++k) {
       pos[k].alloc(((size_t)nn + kPairPad) * sizeof(Vec4<T>));
-      vel[k].alloc((size_t)nn * sizeof(Vec4<T>));
+      vel[k].alloc(((size_t)nn + kPairPad) * sizeof(Vec4<T>));
       id[k].alloc((size_t)nn * sizeof(int32_t));
     }
     cell_of.alloc((size_t)nn * 4);
@@ -733,7 +733,7 @@ struct SimImpl final : SimBase {
       idH.alloc(nn * 4);
       offH.alloc(((size_t)cpad + 1) * 4);
       rs_pos.alloc(((size_t)every * nn + kPairPad) * sizeof(Vec4<T>));
-      rs_vel.alloc((size_t)every * nn * sizeof(Vec4<T>));
+      rs_vel.alloc(((size_t)every * nn + kPairPad) * sizeof(Vec4<T>));
       rs_vnew.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_xpre.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_id.alloc((size_t)every * nn * 4);

@@ -54,6 +54,25 @@ def test_cylinder_steps_bitexact(psx_mod, oracle):
     assert_bitwise(as_dtype(v, "f32"), rv, "vel")
 
 
+@pytest.mark.parametrize("chunk", [4, 2])
+def test_compressed_cylinder_steps_bitexact(psx_mod, oracle, chunk):
+    """The 262,144-sphere cylinder with its heights compressed to 0.6 (vertical spacing 1.02,
+    0.88 contacts per sphere, and the vertical neighbours of every sphere within the prefilter's
+    reach): the pair kernel's rare path runs on most spheres. Binary32, bit-exact against the
+    oracle, with four- and two-wide chunks."""
+    sc = scenes.cylinder_settle()
+    sc.pos[:, 2] = 1.0 + (sc.pos[:, 2] - 1.0) * 0.6
+    steps = 3
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
+        s.set_tuning(use_graph=1, pair_variant=1, pair_chunk=chunk)
+        s.step(steps)
+        p, v = s.state()
+    rp, rv = oracle.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps,
+                        mesh=sc.mesh, dtype="f32", fast=True)
+    assert_bitwise(as_dtype(p, "f32"), rp, "pos")
+    assert_bitwise(as_dtype(v, "f32"), rv, "vel")
+
+
 def test_cylinder_f32_within_1e3_radius_of_f64_oracle_1000_steps(psx_mod, oracle):
     """The headline workload in binary32 over 1,000 steps against the reference algorithm in
     binary64 (the oracle): every sphere within 1e-3 of its radius. The binary64 GPU run of the
