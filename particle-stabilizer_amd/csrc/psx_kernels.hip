@@ -1022,8 +1022,12 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
   }
 }
 
+// The binary32 two-wide form is held to eight waves per SIMD (at most 64 VGPRs; it needs 67
+// unbounded, and spills nothing bounded): it runs on the workloads that fill the GPU many times
+// over, where more resident waves hide more of each wave's chain of slice loads (1 % on the
+// 4.2M-sphere hopper and the 256-scene batch, measured on MI355X).
 template <class T, bool kTable, int W>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
             const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout, const NextBucket next) {
