@@ -75,7 +75,8 @@ class Profiler {
 struct SimBase {
   virtual ~SimBase() = default;
   int64_t n = 0;
-  int64_t ncells = 0, cpad = 0;
+  int64_t ncells = 0, cpad = 0;  // cells, and cells padded to whole scan tiles
+  int64_t id_stride = 0;          // cells per id-slot plane (slot-major id slots)
   int32_t ntiles = 0;
   int64_t tri_count = 0, tcell_entries = 0;
   std::vector<SceneHost> scenes;

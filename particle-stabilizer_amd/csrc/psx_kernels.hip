@@ -464,17 +464,18 @@ __global__ void __launch_bounds__(kScanThreads)
     ovf_ctr[0] = 0u;
   }
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kScanTile);
+  // Only the counts stay in registers across the look-back; the wave scans of the rounds are
+  // redone for the output (a few instructions each) rather than held (two more registers per
+  // round, fewer resident waves).
   uint4 u[kScanRounds];
-  uint32_t tot[kScanRounds], wi[kScanRounds];
   const int w = threadIdx.x >> 6;
   const int lane = lane_id();
 #pragma unroll
   for (int q = 0; q < kScanRounds; ++q) u[q] = src[q * kScanThreads + threadIdx.x];
 #pragma unroll
   for (int q = 0; q < kScanRounds; ++q) {
-    tot[q] = (u[q].x + u[q].y) + (u[q].z + u[q].w);
-    wi[q] = wave_inclusive_scan(tot[q]);
-    if (lane == 63) wsum[q][w] = wi[q];
+    const uint32_t wi = wave_inclusive_scan((u[q].x + u[q].y) + (u[q].z + u[q].w));
+    if (lane == 63) wsum[q][w] = wi;
   }
   __syncthreads();
   uint32_t agg = 0;
@@ -528,7 +529,8 @@ __global__ void __launch_bounds__(kScanThreads)
   uint32_t round_base = s_prefix;  // cells of the sub-tiles before round q
 #pragma unroll
   for (int q = 0; q < kScanRounds; ++q) {
-    uint32_t run = round_base + wi[q] - tot[q];
+    const uint32_t tot = (u[q].x + u[q].y) + (u[q].z + u[q].w);
+    uint32_t run = round_base + wave_inclusive_scan(tot) - tot;
     uint32_t rtot = 0;
 #pragma unroll
     for (int k = 0; k < kWaves; ++k) {

@@ -452,6 +452,7 @@ struct SimImpl final : SimBase {
     ncells = cells;
     cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
     ntiles = (int32_t)(cpad / kScanTile);
+    id_stride = (cells + kIdPlane - 1) / kIdPlane * kIdPlane;
     ntiles_direct = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
 
     // device allocations
@@ -463,7 +464,7 @@ struct SimImpl final : SimBase {
     cell_of.alloc((size_t)nn * 4);
     home.alloc((size_t)nn * 4);
     list.alloc(((size_t)nn + kListPad) * 4);
-    cell_ids.alloc((size_t)cpad * kCellIds * 4);
+    cell_ids.alloc((size_t)id_stride * kCellIds * 4);
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
     count.alloc((size_t)cpad * 4);
@@ -558,7 +559,7 @@ struct SimImpl final : SimBase {
       return launch_integrate<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
                                  scenes_d.as<SceneDev<T>>(), one, id_blocks(), ps, vs, is,
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
-                                 cell_ids.as<int32_t>(), (uint32_t)cpad, ovf.as<int2>(),
+                                 cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
                                  tiles(), mode);
     });
@@ -588,7 +589,7 @@ struct SimImpl final : SimBase {
     launch(2, prof, [&] {
       return launch_rank_gather<T>(stream, nb_s, tuning.block_sort, one ? nullptr : bt_s,
                                    scenes_d.as<SceneDev<T>>(), one, blocks, nn,
-                                   cell_of.as<int32_t>(), off, cell_ids.as<int32_t>(), (uint32_t)cpad,
+                                   cell_of.as<int32_t>(), off, cell_ids.as<int32_t>(), (uint32_t)id_stride,
                                    ovf.as<int2>(), ovf_ctr.as<uint32_t>(), list.as<int32_t>(), ps,
                                    vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
@@ -625,7 +626,7 @@ struct SimImpl final : SimBase {
       nb.cell_of = cell_of.as<int32_t>();
       nb.count = count.as<uint32_t>();
       nb.cell_ids = cell_ids.as<int32_t>();
-      nb.id_stride = (uint32_t)cpad;
+      nb.id_stride = (uint32_t)id_stride;
       nb.ovf = ovf.as<int2>();
       nb.ovf_ctr = ovf_ctr.as<uint32_t>();
       nb.blocks = id_blocks() ? 1 : 0;
