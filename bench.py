@@ -65,6 +65,17 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        for row in open("/proc/cpuinfo"):
+            if row.startswith("model name"):
+                return row.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, steps: int) -> dict:
     from oracle import ref
     threads = min(16, os.cpu_count() or 1)
@@ -74,6 +85,7 @@ def cpu_baseline(sc, steps: int) -> dict:
     dt = time.perf_counter() - t0
     n = sc.radius.shape[0]
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{steps} float64 steps of the same {n:,}-sphere scene from its initial "
                       f"state (reference CPU algorithm: serial pair loop, OpenMP brute-force "
                       f"mesh loop), {dt:.2f} s"}
@@ -233,8 +245,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the path would shard by spatial slabs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the "
+              f"path would shard the cell order into x-slabs, one GPU each, exchanging a layer of "
+              f"boundary cells per step)", file=sys.stderr)
         return 2
 
     import psx

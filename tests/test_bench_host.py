@@ -1,0 +1,44 @@
+"""Host logic of bench.py (no GPU): the algorithmic bytes behind roofline.achieved, the measured
+traffic read from the committed rocprofv3 summary, the CPU description and the one-GPU rule."""
+import subprocess
+import sys
+from pathlib import Path
+
+import bench
+
+REPO = Path(__file__).resolve().parent.parent
+
+
+def test_pair_kernel_bytes_of_the_262k_cylinder():
+    # 262,144 spheres, 48 x 48 x 179 cells, binary32, id slots: the figure quoted per launch in the
+    # committed PMC summary (11 s N for the pair kick, 8 C per offset read and count update, 12 N
+    # for the hash pass)
+    b = bench.algorithmic_bytes(262144, 48 * 48 * 179, 4, True)
+    assert b["k_pairs"] == 21278720
+    assert b["k_scan"] == 12 * 412416 and b["k_scatter_ids"] == 0
+    dense = bench.algorithmic_bytes(262144, 412416, 4, False)
+    assert dense["k_scan"] == 8 * 412416
+    assert dense["k_scatter_ids"] == 16 * 262144 + 8 * 412416
+    assert dense["k_pairs"] == b["k_pairs"] - 4 * 262144  # no id filed in the dense form
+    # binary64 moves twice the scalar bytes of every per-sphere array and the same index bytes
+    b64 = bench.algorithmic_bytes(262144, 412416, 8, True)
+    assert b64["k_pairs"] - b["k_pairs"] == 11 * 4 * 262144
+    assert b64["k_rank_gather"] - b["k_rank_gather"] == 16 * 4 * 262144
+
+
+def test_traffic_read_from_the_committed_summary():
+    text = (REPO / "profiles" / "pmc_k_pairs_cylinder_settle.txt").read_text()
+    line = [r for r in text.splitlines() if r.startswith("traffic_bytes_per_launch:")][0]
+    assert bench.measured_traffic("cylinder_settle", "k_pairs") == float(line.split(":")[1])
+    assert bench.measured_traffic("no_such_workload", "k_pairs") is None
+
+
+def test_cpu_description():
+    assert isinstance(bench.cpu_model(), str) and bench.cpu_model()
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
