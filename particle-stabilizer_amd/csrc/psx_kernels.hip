@@ -573,9 +573,18 @@ __global__ void __launch_bounds__(kDirectThreads)
   }
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kDirectTile);
   const uint4 u = src[threadIdx.x];
-  uint32_t before = 0;  // this thread's share of the earlier tiles' counts
-  for (int32_t j = (int32_t)threadIdx.x; j < tile; j += kDirectThreads)
-    before += tc[(size_t)j * kTileStride];
+  // this thread's share of the earlier tiles' counts: its (at most four) loads are issued
+  // together and summed after (a rolled loop waits for each load before the next)
+  constexpr int kPerThread = kDirectTiles / kDirectThreads;
+  uint32_t share[kPerThread];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int32_t j = (int32_t)threadIdx.x + k * kDirectThreads;
+    share[k] = j < tile ? tc[(size_t)j * kTileStride] : 0u;
+  }
+  uint32_t before = 0;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) before += share[k];
   const uint32_t tot = (u.x + u.y) + (u.z + u.w);
   const uint32_t wi = wave_inclusive_scan(tot);
 #pragma unroll
