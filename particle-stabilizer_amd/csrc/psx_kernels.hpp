@@ -80,9 +80,9 @@ struct NextBucket {
 // Hash and count: the cell of each particle's position after the integration step
 // (kModeIntegrate) or of its current position, counted in its cell (counts must be 0 on entry).
 // blocks: the id is filed into the cell's id slots at its arrival slot (cell_ids: kCellIds slots
-// per cell, slot-major, slot k of cell c at k * id_stride + c with id_stride = C_pad; ovf: n entries
-// (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them and must be 0 on entry, k_scan
-// resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
+// per cell, slot-major, slot k of cell c at k * id_stride + c, id_stride = cells rounded up to
+// kIdPlane; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them and must
+// be 0 on entry, k_scan resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
