@@ -76,19 +76,32 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(sc, steps: int) -> dict:
+def cpu_baseline(sc, steps: int, seconds: float = 0.0) -> dict:
+    """The oracle (the reference CPU algorithm) on the host cores, from the scene's initial state:
+    a run of `steps` steps, then, when `seconds` is set and that run was shorter, one run whose
+    length that first run sizes to about `seconds` of work (the figure is that second run's)."""
     from oracle import ref
     threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    ref.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps, mesh=sc.mesh,
-            dtype="f64", nthreads=threads)
-    dt = time.perf_counter() - t0
+
+    def run(k: int) -> float:
+        t0 = time.perf_counter()
+        ref.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, k, mesh=sc.mesh,
+                dtype="f64", nthreads=threads)
+        return time.perf_counter() - t0
+
+    dt = run(steps)
+    sized = ""
+    if seconds > 0.0 and dt < seconds:
+        first = steps
+        steps = max(steps, int(round(steps * seconds / dt)))
+        dt = run(steps)
+        sized = f"; length from a {first}-step run"
     n = sc.radius.shape[0]
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "cpu": cpu_model(),
             "sample": f"{steps} float64 steps of the same {n:,}-sphere scene from its initial "
                       f"state (reference CPU algorithm: serial pair loop, OpenMP brute-force "
-                      f"mesh loop), {dt:.2f} s"}
+                      f"mesh loop), {dt:.2f} s{sized}"}
 
 
 def _timed(sim, steps: int) -> float:
@@ -230,7 +243,10 @@ def main() -> int:
                     choices=["cylinder_settle", "hopper_settle", "box_settle"])
     ap.add_argument("--dtype", default=None)
     ap.add_argument("--profile-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the CPU baseline (default: a run sized to --cpu-seconds)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate length of the CPU baseline's timed run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
@@ -341,7 +357,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu_steps = args.cpu_steps or {"cylinder_settle": 20, "hopper_settle": 3,
                                        "box_settle": 400}[args.workload]
-        line["cpu_baseline"] = cpu_baseline(sc, cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(sc, cpu_steps,
+                                            0.0 if args.cpu_steps else args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -1,5 +1,6 @@
 """Host logic of bench.py (no GPU): the algorithmic bytes behind roofline.achieved, the measured
-traffic read from the committed rocprofv3 summary, the CPU description and the one-GPU rule."""
+traffic read from the committed rocprofv3 summary, the CPU description, the length of the CPU
+baseline's sample and the one-GPU rule."""
 import subprocess
 import sys
 from pathlib import Path
@@ -42,3 +43,18 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_cpu_baseline_sized_to_the_requested_seconds():
+    """The CPU baseline's timed run is sized by a first run to about the requested seconds; with
+    no target it times the given steps as they are."""
+    from helpers import dense_box
+    sc = dense_box(n_side=6, dtype="f64")
+    fixed = bench.cpu_baseline(sc, 2)
+    assert fixed["sample"].startswith("2 float64 steps") and "length from" not in fixed["sample"]
+    assert fixed["kind"] == "port" and fixed["value"] > 0
+    sized = bench.cpu_baseline(sc, 2, seconds=0.3)
+    steps = int(sized["sample"].split()[0])
+    assert steps > 2 and "length from a 2-step run" in sized["sample"]
+    secs = float(sized["sample"].split("), ")[1].split(" s")[0])
+    assert 0.05 < secs < 3.0
