@@ -255,6 +255,8 @@ def main() -> int:
                     help="sweep the launch shapes again instead of applying the tuning file")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the hopper, batched and adjoint measurements")
+    ap.add_argument("--extras", default="hopper_settle,batched,adjoint",
+                    help="comma-separated extras to run (hopper_settle, batched, adjoint)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="finish with one plain device-to-device copy of this many MiB (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -347,6 +349,8 @@ def main() -> int:
         extras = {}
         for name, fn in (("hopper_settle", bench_hopper), ("batched", bench_batched),
                          ("adjoint", bench_adjoint)):
+            if name not in args.extras.split(","):
+                continue
             try:
                 extras[name] = fn()
             except Exception as e:  # report, keep the headline line
