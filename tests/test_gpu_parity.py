@@ -144,21 +144,24 @@ def test_graph_replay_equals_eager(psx_mod, dtype):
     assert_bitwise(v1, v0, "vel")
 
 
+RUNS = (16, 16, 64, 83, 64, 5, 16)  # 83 = 64 + 16 + 3 single steps
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_graph_replays_interleaved_with_eager_work(psx_mod, oracle, dtype):
-    """The same 16-step graph replayed five times, with the program's own eager work between the
-    replays (state copies, a statistics pass that buckets the state on the same arrays, a
-    synchronisation): every replay's state equals that of an eager handle stepped alongside, bit for
-    bit, on a box scene and a mesh scene, and the 80 steps equal the oracle's (the statistics
-    passes leave the trajectory alone)."""
+    """Graphs replayed repeatedly (16-step and 64-step graphs, and runs that mix both with single
+    steps), with the program's own eager work between the replays (state copies, a statistics pass
+    that buckets the state on the same arrays, a synchronisation): every replay's state equals that
+    of an eager handle stepped alongside, bit for bit, on a box scene and a mesh scene, and the whole
+    run equals the oracle's (the statistics passes leave the trajectory alone)."""
     for sc in (dense_box(dtype=dtype), small_cylinder(n=400, dtype=dtype)):
         with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as e, \
                 psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as g:
             e.set_tuning(use_graph=0)
             g.set_tuning(use_graph=1)
-            for r in range(5):
-                e.step(16)
-                g.step(16)
+            for r, k in enumerate(RUNS):
+                e.step(k)
+                g.step(k)
                 e.synchronize()
                 g.synchronize()
                 pe, ve = e.state()
@@ -168,9 +171,9 @@ def test_graph_replays_interleaved_with_eager_work(psx_mod, oracle, dtype):
                 se, sg = e.statistics()[0], g.statistics()[0]
                 assert se["n_collided"] == sg["n_collided"]
                 assert se["kinetic_energy"] == sg["kinetic_energy"]
-        ref_p, ref_v = oracle_run(oracle, sc, 80)
-        assert_bitwise(as_dtype(pg, dtype), ref_p, "pos after 80 steps")
-        assert_bitwise(as_dtype(vg, dtype), ref_v, "vel after 80 steps")
+        ref_p, ref_v = oracle_run(oracle, sc, sum(RUNS))
+        assert_bitwise(as_dtype(pg, dtype), ref_p, "pos after the runs")
+        assert_bitwise(as_dtype(vg, dtype), ref_v, "vel after the runs")
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
