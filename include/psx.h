@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PSX_ABI_VERSION 3
+#define PSX_ABI_VERSION 4
 
 enum psx_status {
   PSX_OK = 0,
@@ -114,6 +114,12 @@ int psx_destroy(psx_sim* sim);
  * handle's stream and returns without waiting. Statistics are not computed on this path. */
 int psx_step(psx_sim* sim, int64_t nsteps);
 
+/* Captures the hipGraphs that psx_step(sim, nsteps) would replay from the current state, without
+ * running a step (the current state's bucketing hash pass is enqueued, as psx_step would). A run
+ * whose graphs are prepared spends no time capturing or instantiating them. No counterpart in the
+ * reference (its loop launches every kernel from the host, PhysicsEngineCUDA.cpp:25-81). */
+int psx_prepare(psx_sim* sim, int64_t nsteps);
+
 int psx_synchronize(psx_sim* sim);
 
 /* Injects a caller-owned HIP stream (hipStream_t passed as void*); NULL restores the own stream. */
@@ -179,16 +185,20 @@ int psx_statistics(psx_sim* sim, psx_stats* out);
 int psx_set_params(psx_sim* sim, int32_t scene, const psx_params* params);
 
 /* Runs nsteps steps from the current state, keeping a copy of the state every checkpoint_every
- * steps in device memory for psx_window_backward. */
-int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every);
+ * steps in device memory for psx_window_backward. *window_id (may be NULL) receives the window's
+ * id; a handle holds the checkpoints of its latest window only. */
+int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every,
+                       int64_t* window_id);
 
-/* Adjoint of the last forward window. dl_dpos/dl_dvel: dL/d(final positions, velocities), n x 3 of
- * the handle's dtype in device memory, original order. Writes dL/d(initial positions, velocities)
- * to dl_dpos0/dl_dvel0 (same layout) and, for each scene s, dL/d(spring, restitution, gravity_z) to
- * dl_dparams[3s .. 3s+2] (host). Each segment between checkpoints is recomputed forward. The state
- * afterwards is the window's final state. */
-int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, void* dl_dpos0,
-                        void* dl_dvel0, double* dl_dparams);
+/* Adjoint of the forward window window_id, which must be the handle's latest window with no
+ * psx_set_params since (otherwise PSX_ERR_STATE: its checkpoints or coefficients are gone).
+ * dl_dpos/dl_dvel: dL/d(final positions, velocities), n x 3 of the handle's dtype in device memory,
+ * original order. Writes dL/d(initial positions, velocities) to dl_dpos0/dl_dvel0 (same layout)
+ * and, for each scene s, dL/d(spring, restitution, gravity_z) to dl_dparams[3s .. 3s+2] (host).
+ * Each segment between checkpoints is recomputed forward from its checkpoint; the handle's current
+ * state is left as it is. */
+int psx_window_backward(psx_sim* sim, int64_t window_id, const void* dl_dpos, const void* dl_dvel,
+                        void* dl_dpos0, void* dl_dvel0, double* dl_dparams);
 
 /* Device time of the last psx_window_backward spent recomputing segments and in the adjoint steps. */
 int psx_window_stats(const psx_sim* sim, double* replay_ms, double* adjoint_ms);

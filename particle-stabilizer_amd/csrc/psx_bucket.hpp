@@ -12,7 +12,10 @@ __device__ inline int lane_id() { return threadIdx.x & 63; }
 // pre-container position x: its cell and one more member counted there. With id blocks the
 // particle's arrival slot (the counter's old value) places its id in the cell's id slots, slot k of
 // cell c at cell_ids[k * stride + c] (slot-major: the first arrivals of consecutive cells, which
-// consecutive particles are, lie side by side); arrivals past kCellIds go to the overflow list.
+// consecutive particles are, lie side by side). An arrival past kCellIds takes an overflow entry
+// (id, previous entry of its cell) and makes it the head of its cell's chain in the slot plane
+// kCellIds: the cell's last count - kCellIds entries are exactly its overflow members, so a reader
+// follows the chain that many steps from the head and never needs the chain cleared.
 // For the dense lists the count alone is kept (an atomic whose result is not waited for) and the
 // scatter kernel takes the slots by counting back down.
 template <class T>
@@ -37,7 +40,8 @@ __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4
       cell_ids[(size_t)slot * stride + (uint32_t)c] = me;
     } else {  // at most n entries in all
       const uint32_t o = atomicAdd(ovf_ctr, 1u);
-      ovf[o] = make_int2(c, me);
+      const int32_t prev = atomicExch(cell_ids + (size_t)kCellIds * stride + (uint32_t)c, (int32_t)o);
+      ovf[o] = make_int2(me, prev);
     }
   } else {
     cell_of[p] = c;

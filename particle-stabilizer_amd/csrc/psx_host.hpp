@@ -89,6 +89,8 @@ struct SimBase {
   virtual void setup(const std::vector<double>& pos, const std::vector<double>& vel,
                      const std::vector<double>& radius, const std::vector<double>& mass) = 0;
   virtual void step(int64_t nsteps) = 0;
+  // Captures the graphs that step(nsteps) would replay from the current state (no step is run).
+  virtual void prepare(int64_t nsteps) = 0;
   virtual void profile(int64_t nsteps, double* ms, double* per_step) = 0;
   virtual void set_stream(hipStream_t s) = 0;
   virtual void retune(const psx_tuning& t) = 0;
@@ -99,9 +101,10 @@ struct SimBase {
   virtual void debug_bucketing(int32_t* cell, uint32_t* count, uint32_t* offset, int32_t* perm) = 0;
   virtual uint32_t read_flags() = 0;
   virtual void set_params(int32_t scene, const psx_params& p) = 0;
-  virtual void window_forward(int64_t nsteps, int64_t every) = 0;
-  virtual void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0,
-                               double* gparams) = 0;
+  // returns the window's id (psx_window_forward)
+  virtual int64_t window_forward(int64_t nsteps, int64_t every) = 0;
+  virtual void window_backward(int64_t window_id, const void* dlx, const void* dlv, void* dlx0,
+                               void* dlv0, double* gparams) = 0;
   virtual void window_stats(double* replay_ms, double* adjoint_ms) = 0;
   virtual void statistics(psx_stats* out) = 0;
 };

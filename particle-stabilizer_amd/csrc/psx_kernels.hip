@@ -625,7 +625,8 @@ __global__ void __launch_bounds__(kDirectThreads)
 // (ParticleModel.cpp:55-61), so a particle's sorted slot is offset[cell] + the number of cell mates
 // with a smaller id. The mates are read from the cell's first min(count, kCellIds) id slots
 // (slot-major, none for a particle alone in its cell) and, for a cell with more than kCellIds
-// members, from the overflow list.
+// members, from the cell's chain of overflow entries (count - kCellIds of them), so the cost of a
+// particle is bounded by its own cell's occupancy.
 // ---------------------------------------------------------------------------------------------
 template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(kMaxBlockSort<T>)
@@ -677,11 +678,12 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
           for (int k = 0; k < kCellIds; ++k)
             if ((uint32_t)k < cnt) rank += (cell_ids[(size_t)k * id_stride + (uint32_t)c] < me) ? 1u : 0u;
         }
-        if (cnt > (uint32_t)kCellIds) {
-          const uint32_t len = ovf_ctr[1];
-          for (uint32_t o = 0; o < len; ++o) {
+        if (cnt > (uint32_t)kCellIds) {  // the cell's own overflow members, newest first
+          int32_t o = cell_ids[(size_t)kCellIds * id_stride + (uint32_t)c];
+          for (uint32_t k = kCellIds; k < cnt; ++k) {
             const int2 q = ovf[o];
-            rank += (q.x == c && q.y < me) ? 1u : 0u;
+            rank += (q.x < me) ? 1u : 0u;
+            o = q.y;
           }
         }
       } else {  // the first four members in one go (the list carries kListPad spare entries)

@@ -20,7 +20,7 @@ constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 // Cells per id-slot plane are rounded to this rather than to the scan tile, so the id slots take
 // no more memory than the cells need.
 constexpr int64_t kIdPlane = 4096;
-constexpr int kCellIds = 8;                   // id slots per cell; arrivals beyond go to the overflow list
+constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the cell's overflow chain
 constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours
 // Largest workgroup of the rank/gather kernel: its binary64 form (rank, walls and the mesh contact
 // in double) needs more than the 128 registers a 1024-wide workgroup leaves per lane.
@@ -81,8 +81,9 @@ struct NextBucket {
 // (kModeIntegrate) or of its current position, counted in its cell (counts must be 0 on entry).
 // blocks: the id is filed into the cell's id slots at its arrival slot (cell_ids: kCellIds slots
 // per cell, slot-major, slot k of cell c at k * id_stride + c, id_stride = cells rounded up to
-// kIdPlane; ovf: n entries (cell, id) for arrivals past kCellIds; ovf_ctr[0] counts them and must
-// be 0 on entry, k_scan resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
+// kIdPlane, then one plane of per-cell overflow chain heads; ovf: n entries (id, previous entry of
+// the cell) for arrivals past kCellIds; ovf_ctr[0] counts them and must be 0 on entry, k_scan
+// resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,

@@ -263,6 +263,9 @@ struct DevBuf {
   }
 };
 
+// Particles per workgroup of the statistics kernel (its table is built at set-up).
+constexpr int kStatBlock = 256;
+
 static const char* kKernelNames[kNumKernels] = {"k_integrate", "k_scan", "k_rank_gather",
                                                 "k_pairs", "k_scatter_ids"};
 
@@ -469,7 +472,7 @@ struct SimImpl final : SimBase {
     cell_of.alloc((size_t)nn * 4);
     home.alloc((size_t)nn * 4);
     list.alloc(((size_t)nn + kListPad) * 4);
-    cell_ids.alloc((size_t)id_stride * kCellIds * 4);
+    cell_ids.alloc((size_t)id_stride * (kCellIds + 1) * 4);  // id slots + overflow chain heads
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
     count.alloc((size_t)cpad * 4);
@@ -516,6 +519,7 @@ struct SimImpl final : SimBase {
     table(tuning.block_integrate, &nb);
     table(tuning.block_sort, &nb);
     table(pair_particles_per_block(), &nb);
+    table(kStatBlock, &nb);
     PSX_HIP(hipDeviceSynchronize());
   }
 
@@ -703,6 +707,9 @@ struct SimImpl final : SimBase {
   DevBuf lxA, lvA, lxB, lvB, gk, ge, gg, gout;
   std::map<int64_t, hipGraphExec_t> g_replay, g_adjoint;
   double win_replay_ms = 0.0, win_adjoint_ms = 0.0;
+  // Ids of forward windows: each psx_window_forward takes the next one; win_id names the window
+  // whose checkpoints are held (0: none, or its coefficients have changed since).
+  int64_t win_counter = 0, win_id = 0;
 
   void destroy_window_graphs() {
     for (auto* m : {&g_replay, &g_adjoint}) {
@@ -725,7 +732,7 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpyAsync(di, ck_id.as<int32_t>() + slot_k * nn, nn * 4, hipMemcpyDeviceToDevice, stream));
   }
 
-  void window_forward(int64_t nsteps, int64_t every) override {
+  int64_t window_forward(int64_t nsteps, int64_t every) override {
     const size_t nn = (size_t)n;
     const int64_t nseg = (nsteps + every - 1) / every;
     if (nseg != win_nseg || every != win_every) {
@@ -752,11 +759,14 @@ struct SimImpl final : SimBase {
     win_steps = nsteps;
     win_every = every;
     win_nseg = nseg;
+    win_id = 0;  // the checkpoints are being replaced
     for (int64_t sgm = 0; sgm < nseg; ++sgm) {
       save_state(sgm);
       step(std::min(every, nsteps - sgm * every));
     }
     save_state(nseg);
+    win_id = ++win_counter;
+    return win_id;
   }
 
   // pre-step state of recorded step k
@@ -840,19 +850,37 @@ struct SimImpl final : SimBase {
     return ge_;
   }
 
+  // The graph of fn(ks), captured and instantiated on first use (its own call, so that a failure
+  // names the capture or the launch).
+  __attribute__((noinline)) hipGraphExec_t cached_graph(std::map<int64_t, hipGraphExec_t>& cache,
+                                                        int64_t ks, void (SimImpl::*fn)(int64_t)) {
+    auto it = cache.find(ks);
+    if (it == cache.end()) it = cache.emplace(ks, capture_graph([&] { (this->*fn)(ks); })).first;
+    return it->second;
+  }
+
   void run_cached(std::map<int64_t, hipGraphExec_t>& cache, int64_t ks, void (SimImpl::*fn)(int64_t)) {
     if (!tuning.use_graph) {
       (this->*fn)(ks);
       return;
     }
-    auto it = cache.find(ks);
-    if (it == cache.end()) it = cache.emplace(ks, capture_graph([&] { (this->*fn)(ks); })).first;
-    PSX_HIP(hipGraphLaunch(it->second, stream));
+    const hipGraphExec_t ge_ = cached_graph(cache, ks, fn);
+    const hipError_t e = hipGraphLaunch(ge_, stream);
+    if (e != hipSuccess)
+      throw Error(PSX_ERR_HIP, std::string("hipGraphLaunch (window segment of ") + std::to_string(ks) +
+                                   " steps): " + hipGetErrorString(e));
   }
 
-  void window_backward(const void* dlx, const void* dlv, void* dlx0, void* dlv0, double* gparams) override {
+  void window_backward(int64_t window_id, const void* dlx, const void* dlv, void* dlx0, void* dlv0,
+                       double* gparams) override {
+    if (win_nseg <= 0 || win_id == 0)
+      throw Error(PSX_ERR_STATE, "psx_window_backward needs a psx_window_forward first (and no "
+                                 "psx_set_params since)");
+    if (window_id != win_id)
+      throw Error(PSX_ERR_STATE, "psx_window_backward: window " + std::to_string(window_id) +
+                                     " is stale; the checkpoints held are those of window " +
+                                     std::to_string(win_id));
     drop_hash();  // the recompute buckets from zero counts
-    if (win_nseg <= 0) throw Error(PSX_ERR_STATE, "psx_window_backward needs a psx_window_forward first");
     const size_t nn = (size_t)n;
     PSX_HIP(hipMemcpyAsync(lxA.p, dlx, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
     PSX_HIP(hipMemcpyAsync(lvA.p, dlv, 3 * nn * sizeof(T), hipMemcpyDeviceToDevice, stream));
@@ -896,11 +924,6 @@ struct SimImpl final : SimBase {
       win_replay_ms += a;
       win_adjoint_ms += b;
     }
-    // back to the window's final state
-    load_state(win_nseg, pos[0].p, vel[0].p, id[0].p);
-    cur_pos = 0;
-    cur_vel = 0;
-    PSX_HIP(hipStreamSynchronize(stream));
   }
 
   void window_stats(double* replay_ms, double* adjoint_ms) override {
@@ -915,6 +938,7 @@ struct SimImpl final : SimBase {
         throw Error(PSX_ERR_INVALID, "psx_set_params cannot change the container bounds");
     if (p.dtype != sh.prm.dtype) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change dtype");
     sh.prm = p;
+    win_id = 0;   // a held window was run with the old coefficients
     drop_hash();  // hashed with the old dt
     PSX_HIP(hipStreamSynchronize(stream));
     SceneDev<T>& d = scenes_h[(size_t)scene];
@@ -978,6 +1002,18 @@ struct SimImpl final : SimBase {
       }
     }
     for (; left > 0; --left) step_once(nullptr);
+  }
+
+  // A run of nsteps from the current state replays graphs of the current buffer parity only (both
+  // graph lengths are even): capture those it will use.
+  void prepare(int64_t nsteps) override {
+    if (nsteps <= 0 || !tuning.use_graph) return;
+    ensure_hashed(nullptr);
+    int64_t left = nsteps;
+    for (int li = 0; li < 2; ++li) {
+      if (left >= kGraphSteps[li] && !graphs[li][cur_pos][cur_vel]) capture(li, cur_pos, cur_vel);
+      left %= kGraphSteps[li];
+    }
   }
 
   void profile(int64_t nsteps, double* ms, double* per_step) override {
@@ -1310,6 +1346,15 @@ int psx_step(psx_sim* sim, int64_t nsteps) {
   })
 }
 
+int psx_prepare(psx_sim* sim, int64_t nsteps) {
+  if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
+  if (nsteps < 0) return set_error(PSX_ERR_INVALID, "nsteps must be >= 0");
+  PSX_TRY({
+    sim->impl->prepare(nsteps);
+    return PSX_OK;
+  })
+}
+
 int psx_synchronize(psx_sim* sim) {
   if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
   PSX_TRY({
@@ -1441,21 +1486,23 @@ int psx_set_params(psx_sim* sim, int32_t scene, const psx_params* params) {
   })
 }
 
-int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every) {
+int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every,
+                       int64_t* window_id) {
   if (!sim) return set_error(PSX_ERR_INVALID, "sim is NULL");
   if (nsteps <= 0 || checkpoint_every <= 0) return set_error(PSX_ERR_INVALID, "nsteps and checkpoint_every must be positive");
   PSX_TRY({
-    sim->impl->window_forward(nsteps, std::min(nsteps, checkpoint_every));
+    const int64_t id = sim->impl->window_forward(nsteps, std::min(nsteps, checkpoint_every));
+    if (window_id) *window_id = id;
     return PSX_OK;
   })
 }
 
-int psx_window_backward(psx_sim* sim, const void* dl_dpos, const void* dl_dvel, void* dl_dpos0,
-                        void* dl_dvel0, double* dl_dparams) {
+int psx_window_backward(psx_sim* sim, int64_t window_id, const void* dl_dpos, const void* dl_dvel,
+                        void* dl_dpos0, void* dl_dvel0, double* dl_dparams) {
   if (!sim || !dl_dpos || !dl_dvel || !dl_dpos0 || !dl_dvel0)
     return set_error(PSX_ERR_INVALID, "NULL argument");
   PSX_TRY({
-    sim->impl->window_backward(dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams);
+    sim->impl->window_backward(window_id, dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams);
     return PSX_OK;
   })
 }

@@ -67,6 +67,7 @@ _SIGNATURES = {
                                    C.POINTER(C.c_void_p)]),
     "psx_destroy": (C.c_int, [C.c_void_p]),
     "psx_step": (C.c_int, [C.c_void_p, C.c_int64]),
+    "psx_prepare": (C.c_int, [C.c_void_p, C.c_int64]),
     "psx_synchronize": (C.c_int, [C.c_void_p]),
     "psx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "psx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(psx_tuning)]),
@@ -88,10 +89,10 @@ _SIGNATURES = {
     "psx_profile": (C.c_int, [C.c_void_p, C.c_int64, _DP, _DP, C.c_int32, _I32P]),
     "psx_kernel_name": (C.c_char_p, [C.c_int32]),
     "psx_set_params": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(psx_params)]),
-    "psx_window_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "psx_window_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _I64P]),
     "psx_window_stats": (C.c_int, [C.c_void_p, _DP, _DP]),
-    "psx_window_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      _DP]),
+    "psx_window_backward": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, _DP]),
     "psx_read_csv": (C.c_int, [C.c_char_p, C.c_double, _DP, _I64P, C.POINTER(_DP), C.POINTER(_DP),
                                C.POINTER(_DP), C.POINTER(_DP), _I64P]),
     "psx_write_csv": (C.c_int, [C.c_char_p, C.c_int64, _DP, _DP]),

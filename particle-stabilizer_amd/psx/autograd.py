@@ -39,10 +39,13 @@ class _Window(torch.autograd.Function):
         vN = torch.empty_like(v0c)
         _torch_done(dev)
         sim.set_state_device(x0c.data_ptr(), v0c.data_ptr())
-        sim.window_forward(nsteps, every)
+        wid = sim.window_forward(nsteps, every)
         sim.get_state_device(xN.data_ptr(), vN.data_ptr())
         sim.synchronize()
         ctx.sim = sim
+        # the handle keeps the checkpoints of its latest window only: a backward after another
+        # window (or a coefficient change) on the same handle is refused, not silently wrong
+        ctx.window_id = wid
         ctx.dtypes = (x0.dtype, v0.dtype)
         return xN, vN
 
@@ -57,7 +60,8 @@ class _Window(torch.autograd.Function):
         gx0 = torch.empty_like(gx)
         gv0 = torch.empty_like(gv)
         _torch_done(gx.device)
-        gp = sim.window_backward(gx.data_ptr(), gv.data_ptr(), gx0.data_ptr(), gv0.data_ptr())[0]
+        gp = sim.window_backward(ctx.window_id, gx.data_ptr(), gv.data_ptr(), gx0.data_ptr(),
+                                 gv0.data_ptr())[0]
         sim.synchronize()
         t = lambda v: torch.tensor(v, dtype=torch.float64, device=gx.device)  # noqa: E731
         return (gx0.to(ctx.dtypes[0]), gv0.to(ctx.dtypes[1]), t(gp[0]), t(gp[1]), t(gp[2]),
@@ -97,16 +101,24 @@ class CalibrationResult:
     spring: float
     restitution: float
     gravity_z: float
+    v0: np.ndarray
     losses: list
     grads: list
+    best_loss: float
 
 
 def calibrate(sim: Simulation, x0: np.ndarray, v0: np.ndarray, target_hist: torch.Tensor,
               target_pos: torch.Tensor | None, nsteps: int, spring: float, restitution: float,
-              gravity_z: float, iters: int = 10, lr: float = 0.05, lo: float = 0.0,
-              hi: float = 30.0, checkpoint_every: int = 100) -> CalibrationResult:
-    """Gradient descent (Adam on log-spring, restitution, gravity_z and v0) against a target
-    packing, with the window's adjoint supplying every gradient."""
+              gravity_z: float, iters: int = 10, lr: float = 0.05, lr_v0: float | None = None,
+              lo: float = 0.0, hi: float = 30.0, checkpoint_every: int = 100) -> CalibrationResult:
+    """Fits spring, restitution, gravity_z and the initial velocities against a target packing
+    (packing_loss), the window's adjoint supplying every gradient: Adam on (log spring,
+    restitution, gravity_z) with step lr and on v0 with step lr_v0 (default lr / 10).
+
+    Contact dynamics make the loss piecewise smooth and, over long windows, rough: a step that
+    raises the loss is undone and both steps are halved (the gradient at the best point is a
+    descent direction for a small enough step), so the best loss never rises. Every evaluated loss
+    is returned in `losses`, the parameters of the best one in the result."""
     dev = torch.device("cuda")
     dt = torch.float32 if sim.dtype == np.float32 else torch.float64
     x0t = torch.tensor(x0, dtype=dt, device=dev)
@@ -114,16 +126,38 @@ def calibrate(sim: Simulation, x0: np.ndarray, v0: np.ndarray, target_hist: torc
     logk = torch.tensor(np.log(spring), dtype=torch.float64, device=dev, requires_grad=True)
     es = torch.tensor(restitution, dtype=torch.float64, device=dev, requires_grad=True)
     gz = torch.tensor(gravity_z, dtype=torch.float64, device=dev, requires_grad=True)
-    opt = torch.optim.Adam([logk, es, gz, v0t], lr=lr)
+    params = [logk, es, gz, v0t]
+    steps = [lr, lr, lr, lr / 10.0 if lr_v0 is None else lr_v0]
+
+    def make_opt():
+        return torch.optim.Adam([{"params": [q], "lr": a} for q, a in zip(params, steps)])
+
+    opt = make_opt()
     losses, grads = [], []
+    best, best_loss = None, float("inf")
     for _ in range(iters):
         opt.zero_grad()
         xN, _ = window(sim, x0t, v0t, torch.exp(logk), es, gz, nsteps, checkpoint_every)
         loss = packing_loss(xN.to(torch.float64), target_hist, target_pos, lo, hi)
         loss.backward()
-        losses.append(float(loss))
+        lv = float(loss)
+        losses.append(lv)
         grads.append((float(logk.grad), float(es.grad), float(gz.grad)))
+        if lv < best_loss:
+            best_loss = lv
+            best = [(q.detach().clone(), q.grad.detach().clone()) for q in params]
+        else:  # undo: back to the best point with its gradient, half the steps
+            steps = [a * 0.5 for a in steps]
+            opt = make_opt()
+            with torch.no_grad():
+                for q, (val, g) in zip(params, best):
+                    q.copy_(val)
+                    q.grad = g.clone()
         opt.step()
         with torch.no_grad():
             es.clamp_(0.0, 1.0)
-    return CalibrationResult(float(torch.exp(logk)), float(es), float(gz), losses, grads)
+    with torch.no_grad():
+        for q, (val, _) in zip(params, best):
+            q.copy_(val)
+    return CalibrationResult(float(torch.exp(logk)), float(es), float(gz),
+                             v0t.detach().double().cpu().numpy(), losses, grads, best_loss)

@@ -128,9 +128,12 @@ def batched_boxes(n_scenes: int = 256, seed: int = 0, dtype: str = "f32",
 
 
 def calibration_scene(n_xy: int = 32, n_z: int = 16, seed: int = 0, dtype: str = "f32",
-                      spring: float = 1e4, restitution: float = 0.5) -> Scene:
+                      spring: float = 1e4, restitution: float = 0.5, v0z: float = -5.0) -> Scene:
     """n_xy x n_xy x n_z lattice (spacing 1.2, jitter +-0.05) in an 80-triangle open box mesh
-    (0,0,0)-(40,40,60); r ~ U[0.45, 0.55]; g (0,0,-9.8); dt 1e-4."""
+    (0,0,0)-(40,40,60); r ~ U[0.45, 0.55]; g (0,0,-9.8); dt 1e-4; initial velocity (0, 0, v0z).
+    The lattice starts 0.4-0.5 above the floor: from rest it would not reach it within a
+    2,000-step window (0.2 s), and spring and restitution would have no contact to act on; at
+    -5 it lands after about 900 steps and the layers above pile onto it for the rest."""
     rng = np.random.default_rng(seed)
     n = n_xy * n_xy * n_z
     pos = _lattice_box(n, (1.0, 1.0, 1.0), 1.2, n_xy, n_xy, rng, 0.05)
@@ -139,5 +142,7 @@ def calibration_scene(n_xy: int = 32, n_z: int = 16, seed: int = 0, dtype: str =
     mesh = meshes.box_mesh((0.0, 0.0, 0.0), (edge, edge, 60.0))
     prm = Params(gravity=(0.0, 0.0, -9.8), walls=(0.5,) * 6, spring=spring,
                  restitution=restitution, dt=1e-4, dtype=dtype)
-    return Scene(pos, np.zeros((n, 3)), r, prm, mass_of(r), mesh,
+    vel = np.zeros((n, 3))
+    vel[:, 2] = v0z
+    return Scene(pos, vel, r, prm, mass_of(r), mesh,
                  meta={"name": "calibration_scene", "seed": seed, "generator": GENERATOR})

@@ -135,6 +135,10 @@ class Simulation:
     def step(self, nsteps: int = 1) -> None:
         check(self._lib.psx_step(self._h, int(nsteps)))
 
+    def prepare(self, nsteps: int) -> None:
+        """Captures the graphs a later step(nsteps) replays from the current state (no step)."""
+        check(self._lib.psx_prepare(self._h, int(nsteps)))
+
     def synchronize(self) -> None:
         check(self._lib.psx_synchronize(self._h))
 
@@ -213,15 +217,19 @@ class Simulation:
         if scene == 0:
             self.params = params
 
-    def window_forward(self, nsteps: int, checkpoint_every: int = 100) -> None:
-        check(self._lib.psx_window_forward(self._h, int(nsteps), int(checkpoint_every)))
+    def window_forward(self, nsteps: int, checkpoint_every: int = 100) -> int:
+        """Runs a window of nsteps steps with checkpoints; returns the window's id."""
+        wid = C.c_int64()
+        check(self._lib.psx_window_forward(self._h, int(nsteps), int(checkpoint_every),
+                                           C.byref(wid)))
+        return wid.value
 
-    def window_backward(self, dl_dpos_ptr: int, dl_dvel_ptr: int, out_pos_ptr: int,
-                        out_vel_ptr: int) -> np.ndarray:
-        """Device pointers (n x 3, the handle's dtype); returns dL/d(spring, restitution, g_z)
-        per scene as an (n_scenes, 3) array."""
+    def window_backward(self, window_id: int, dl_dpos_ptr: int, dl_dvel_ptr: int,
+                        out_pos_ptr: int, out_vel_ptr: int) -> np.ndarray:
+        """Adjoint of window `window_id` (the latest one). Device pointers (n x 3, the handle's
+        dtype); returns dL/d(spring, restitution, g_z) per scene as an (n_scenes, 3) array."""
         g = np.zeros(3 * self.num_scenes)
-        check(self._lib.psx_window_backward(self._h, C.c_void_p(dl_dpos_ptr),
+        check(self._lib.psx_window_backward(self._h, int(window_id), C.c_void_p(dl_dpos_ptr),
                                             C.c_void_p(dl_dvel_ptr), C.c_void_p(out_pos_ptr),
                                             C.c_void_p(out_vel_ptr), _dptr(g)))
         return g.reshape(-1, 3)

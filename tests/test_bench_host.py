@@ -11,11 +11,11 @@ REPO = Path(__file__).resolve().parent.parent
 
 
 def test_pair_kernel_bytes_of_the_262k_cylinder():
-    # 262,144 spheres, 48 x 48 x 179 cells, binary32, id slots: the figure quoted per launch in the
-    # committed PMC summary (11 s N for the pair kick, 8 C per offset read and count update, 12 N
-    # for the hash pass)
+    # 262,144 spheres, 48 x 48 x 179 cells, binary32, id slots: 11 s N for the pair kick, 8 C for
+    # the offset reads, 12 N for the hash pass's ids and cells and 8 N for its count updates (one
+    # count read and written per sphere, not every cell of the grid)
     b = bench.algorithmic_bytes(262144, 48 * 48 * 179, 4, True)
-    assert b["k_pairs"] == 21278720
+    assert b["k_pairs"] == 44 * 262144 + 8 * 412416 + 20 * 262144 == 20076544
     assert b["k_scan"] == 12 * 412416 and b["k_scatter_ids"] == 0
     dense = bench.algorithmic_bytes(262144, 412416, 4, False)
     assert dense["k_scan"] == 8 * 412416
@@ -25,6 +25,23 @@ def test_pair_kernel_bytes_of_the_262k_cylinder():
     b64 = bench.algorithmic_bytes(262144, 412416, 8, True)
     assert b64["k_pairs"] - b["k_pairs"] == 11 * 4 * 262144
     assert b64["k_rank_gather"] - b["k_rank_gather"] == 16 * 4 * 262144
+
+
+def test_pair_kernel_bytes_of_the_hopper():
+    # 4,194,304 spheres on the 179 x 179 x 358 grid: the per-launch figure behind the headline
+    # roofline (64 N + 8 C)
+    n, c = 4194304, 179 * 179 * 358
+    assert bench.algorithmic_bytes(n, c, 4, True)["k_pairs"] == 64 * n + 8 * c == 360200880
+
+
+def test_cpu_threads_follow_the_cpu_share(monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    t, host = bench.cpu_threads()
+    assert t == min(3, len(__import__("os").sched_getaffinity(0)))
+    assert host["nproc"] >= 1 and host["affinity_cores"] >= t and host["model"]
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    t, host = bench.cpu_threads()
+    assert t == host["affinity_cores"] and host["omp_num_threads"] is None
 
 
 def test_traffic_read_from_the_committed_summary():

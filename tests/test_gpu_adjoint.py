@@ -36,7 +36,7 @@ def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10, graph=False):
     out = {"x0": x0.grad.double().cpu().numpy(), "v0": v0.grad.double().cpu().numpy(),
            "k": float(k.grad), "es": float(es.grad), "gz": float(gz.grad),
            "loss": float(loss.detach())}
-    # after the backward the handle holds the window's final state again
+    # the backward leaves the handle's state (the window's final state) as it is
     p, v = sim.state()
     assert np.array_equal(p, xN.detach().double().cpu().numpy())
     assert np.array_equal(v, vN.detach().double().cpu().numpy())

@@ -387,3 +387,121 @@ def test_fast_mode_equals_reference_loops(oracle, dtype):
                        mesh=sc.mesh, dtype=dtype, fast=True)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
         assert not np.array_equal(a[0], np.asarray(sc.pos, a[0].dtype))
+
+
+# ---------------------------------------------------------------------------------------------
+# the points where the reference's CPU and CUDA engines differ (PhysicsEngine.cpp:38-48 against
+# PhysicsEngineCUDA.cpp:36-43): each closed form below holds for the CPU engine and fails for the
+# CUDA engine's variant
+# ---------------------------------------------------------------------------------------------
+def test_spring_kick_carries_no_mass_factor(oracle):
+    """ParticleModel.cpp:87-89: the spring term is k * delta * normalize(d), whatever the masses
+    (the CUDA kernel divides it by m_i, ParticleModel_kernels.cu:195). With masses 0.3 and 7 both
+    spheres receive kicks of the same size k * delta, in opposite directions."""
+    k = 250.0
+    m = (0.3, 7.0)
+    p, v, vn, nrm, d = _two_body(oracle, es=0.0, k=k, m=m)
+    delta = 1.0 - np.linalg.norm(d)
+    assert delta > 0.05
+    assert np.allclose(vn[0] - v[0], k * delta * nrm, rtol=1e-13, atol=1e-12)
+    assert np.allclose(vn[1] - v[1], -k * delta * nrm, rtol=1e-13, atol=1e-12)
+    # the mass-divided variant would differ by factors 1/0.3 and 1/7
+    assert not np.allclose(vn[0] - v[0], k * delta * nrm / m[0], rtol=1e-3)
+
+
+def _tilted_wall(n2):
+    """Two large triangles through the origin with outward normal n2 (in the x-z plane)."""
+    u = np.array([0.0, 1.0, 0.0])
+    w = np.cross(u, n2)  # in the plane, perpendicular to u
+    c = [s1 * 10.0 * u + s2 * 10.0 * w for s1, s2 in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+    tri = np.array([np.concatenate([c[0], c[1], c[2]]), np.concatenate([c[0], c[2], c[3]])])
+    return tri
+
+
+def _crease_60(oracle):
+    """Floor z = 0 (normal -z, triangles 0 and 1) and a wall through the origin whose outward
+    normal n2 = (-sqrt(3)/2, 0, -1/2) makes 60 degrees with the floor's (triangles 2 and 3): a
+    120-degree wedge along the y axis."""
+    n2 = np.array([-math.sqrt(3.0) / 2.0, 0.0, -0.5])
+    wall = _tilted_wall(n2)
+    nw = oracle.mesh_normals(wall)
+    if not np.allclose(nw, n2):
+        wall = wall.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9)
+    assert np.allclose(oracle.mesh_normals(wall), n2)
+    return np.concatenate([_floor(), wall]), n2
+
+
+def test_mesh_corrections_are_sequential(oracle):
+    """PolygonContainer.cpp:206-210: each hit moves the running position y so that it ends r
+    inside that triangle's plane, starting from where the previous hit left it (the CUDA kernels
+    sum corrections computed from the original x, PolygonContainerCUDA_kernels.cu:261-294). For a
+    sphere at x = (x0, 0, z0) penetrating the floor, then the 60-degree wall:
+      floor hit:  y = (x0, 0, r)
+      wall hit:   m = r - (sqrt(3)/2 x0 + r/2),  y = (x0 + sqrt(3)/2 m, 0, r + m/2)
+    so the sphere ends exactly r inside the wall and more than r above the floor."""
+    tri, n2 = _crease_60(oracle)
+    r, e = 0.5, 0.5
+    x0, z0 = 0.1, 0.3
+    prm = Params(gravity=(0, 0, 0), walls=(e,) * 6, dt=2.0 ** -40, spring=0.0, restitution=0.0)
+    v0 = np.array([[0.25, 0.5, -1.0]])
+    p, v = oracle.run(prm.as_array(), np.array([[x0, 0.0, z0]]), v0, [r], [1.0], 1, mesh=tri)
+    m = r - (math.sqrt(3.0) / 2.0 * x0 + r / 2.0)
+    seq = np.array([x0 + math.sqrt(3.0) / 2.0 * m, 0.0, r + m / 2.0])
+    assert np.allclose(p[0], seq, atol=1e-9)
+    # the summed corrections of the CUDA kernels would land elsewhere
+    s2 = math.sqrt(3.0) / 2.0 * x0 + z0 / 2.0
+    summed = np.array([x0, 0.0, z0]) + (r - z0) * np.array([0, 0, 1.0]) - (r - s2) * n2
+    assert np.linalg.norm(summed - seq) > 0.05
+    # distance r to the wall's plane, more than r above the floor
+    assert abs(-(p[0] @ n2) - r) < 1e-9 and p[0, 2] > r + 0.05
+    # velocity: the average of the two reflections times e
+    n1 = np.array([0.0, 0.0, -1.0])
+    refl = lambda vv, nn: vv - 2.0 * (vv @ nn) * nn  # noqa: E731
+    assert np.allclose(v[0], e * (refl(v0[0], n1) + refl(v0[0], n2)) / 2.0, rtol=1e-14)
+
+
+def test_mesh_contact_tested_at_the_original_position(oracle):
+    """PolygonContainer.cpp:184-193: whether a triangle is hit is decided from the original x,
+    not from the running position. At x = (0.4, 0, 0.1) the floor hit lifts y to (0.4, 0, r),
+    which is more than r from the wall's plane; the wall still counts as hit (tested at x), so its
+    correction m = r - (sqrt(3)/2 0.4 + r/2) < 0 pulls y back to r from the wall, and the velocity
+    is the average of both reflections."""
+    tri, n2 = _crease_60(oracle)
+    r, e = 0.5, 0.5
+    x0, z0 = 0.4, 0.1
+    prm = Params(gravity=(0, 0, 0), walls=(e,) * 6, dt=2.0 ** -40, spring=0.0, restitution=0.0)
+    v0 = np.array([[-0.75, 0.25, -2.0]])
+    p, v = oracle.run(prm.as_array(), np.array([[x0, 0.0, z0]]), v0, [r], [1.0], 1, mesh=tri)
+    m = r - (math.sqrt(3.0) / 2.0 * x0 + r / 2.0)
+    assert m < 0.0
+    assert np.allclose(p[0], [x0 + math.sqrt(3.0) / 2.0 * m, 0.0, r + m / 2.0], atol=1e-9)
+    n1 = np.array([0.0, 0.0, -1.0])
+    refl = lambda vv, nn: vv - 2.0 * (vv @ nn) * nn  # noqa: E731
+    assert np.allclose(v[0], e * (refl(v0[0], n1) + refl(v0[0], n2)) / 2.0, rtol=1e-14)
+    # a test at the running position would have counted the floor only
+    assert not np.allclose(v[0], e * refl(v0[0], n1), rtol=1e-3)
+
+
+def test_bucketing_before_walls(oracle):
+    """PhysicsEngine.cpp:40-47: the cell lists are built from the positions before the container
+    acts, while each sphere's neighbour search starts from the cell of its position after it
+    (ParticleModel.cpp:106-110); the CUDA engine does both after the walls. Cells are 1.05 wide
+    (r = 0.5) and the box ends at x = 10.6, so cell 9 spans [9.45, 10.5) and cell 10 [10.5, 11.55).
+    Sphere A starts outside the box at x = 10.7 (listed in cell 10) and the wall puts it at 10.1
+    (home cell 9); sphere B rests at x = 9.3 (cell 8), 0.8 from A's new position: in contact.
+    A's search (cells 8-10) finds B and A is kicked by k * delta along +x; B's search (cells 7-9)
+    does not find A, which is listed in cell 10, so B is not kicked at all."""
+    k = 2.0
+    prm = Params(gravity=(0, 0, 0), walls=(0.5,) * 6, dt=2.0 ** -40, box_min=(0, 0, 0),
+                 box_max=(10.6, 10.0, 10.0), spring=k, restitution=0.0)
+    h, n3 = oracle.grid(0.5, prm.box_min, prm.box_max)
+    assert n3[0] == 11
+    assert oracle.axis_index(10.7, 0.0, h, 11) == 10 and oracle.axis_index(10.1, 0.0, h, 11) == 9
+    assert oracle.axis_index(9.3, 0.0, h, 11) == 8
+    x = np.array([[10.7, 5.0, 5.0], [9.3, 5.0, 5.0]])
+    v = np.zeros((2, 3))
+    p, vn = oracle.run(prm.as_array(), x, v, [0.5, 0.5], [1.0, 1.0], 1)
+    assert p[0, 0] == 10.6 - 0.5 and p[1, 0] == 9.3
+    delta = 1.0 - (p[0, 0] - p[1, 0])
+    assert np.allclose(vn[0], [k * delta, 0.0, 0.0], rtol=1e-12)
+    assert np.array_equal(vn[1], [0.0, 0.0, 0.0])
