@@ -175,6 +175,15 @@ typedef struct psx_stats {
  * not the reference's sequential rounding). Waits for the stream. */
 int psx_statistics(psx_sim* sim, psx_stats* out);
 
+/* psx_statistics without any allocation, for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Its
+ * per-workgroup partials and per-scene totals live in a fixed 4 MiB device array that each GPU
+ * running the library holds until the process ends, so one call takes at most about 14 million
+ * particles (PSX_ERR_INVALID beyond); the totals are combined on the device in psx_statistics'
+ * order and equal its results bit for bit. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads, other handles) race. Waits for the stream. */
+int psx_statistics_noalloc(psx_sim* sim, psx_stats* out);
+
 /* ---- differentiable windows ---------------------------------------------------------------- */
 /* No counterpart in the reference. Gradients of a loss on the state after a window of steps with
  * respect to the initial state and to spring, sphere restitution and gravity_z, holding each step's

@@ -107,6 +107,9 @@ struct SimBase {
                                void* dlv0, double* gparams) = 0;
   virtual void window_stats(double* replay_ms, double* adjoint_ms) = 0;
   virtual void statistics(psx_stats* out) = 0;
+  // statistics() with its per-workgroup partials and per-scene totals in the caller's device
+  // buffer `scratch` of `bytes` bytes (no allocation); the totals are copied to out.
+  virtual void statistics_into(psx_stats* out, void* scratch, size_t bytes) = 0;
 };
 
 std::unique_ptr<SimBase> make_sim(int dtype);

@@ -1323,6 +1323,42 @@ __global__ void __launch_bounds__(256)
   });
 }
 
+// Per-scene totals of the workgroup partials, one thread per scene, in workgroup order: the same
+// operations in the same order as the host's combination of the partials (running sums in
+// binary64, std::min / std::max as (b < a) ? b : a and (a < b) ? b : a), so the two agree bit for
+// bit. A scene's workgroups are the contiguous run of table entries naming it (found by binary
+// search over the ascending scene numbers).
+__global__ void __launch_bounds__(64)
+    k_stats_combine(int32_t nscenes, int32_t ntab, const BlockInfo* __restrict__ btab,
+                    const StatPart* __restrict__ part, psx_stats* __restrict__ out) {
+  const int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= nscenes) return;
+  int32_t lo = 0, hi = ntab;  // first entry with scene >= s
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (btab[mid].scene < s) lo = mid + 1; else hi = mid;
+  }
+  psx_stats r;
+  r.kinetic_energy = 0.0;
+  r.n_collided = 0;
+  r.max_overlap = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    r.min[a] = INFINITY;
+    r.max[a] = -INFINITY;
+  }
+  for (int32_t k = lo; k < ntab && btab[k].scene == s; ++k) {
+    const StatPart q = part[k];
+    r.kinetic_energy += q.ke;
+    r.n_collided += (int64_t)q.n_collided;
+    r.max_overlap = (r.max_overlap < q.max_overlap) ? q.max_overlap : r.max_overlap;
+    for (int a = 0; a < 3; ++a) {
+      r.min[a] = (q.mn[a] < r.min[a]) ? q.mn[a] : r.min[a];
+      r.max[a] = (r.max[a] < q.mx[a]) ? q.mx[a] : r.max[a];
+    }
+  }
+  out[s] = r;
+}
+
 // ---------------------------------------------------------------------------------------------
 // state in/out (original order <-> sorted order)
 // ---------------------------------------------------------------------------------------------
@@ -1536,6 +1572,13 @@ hipError_t launch_stats(hipStream_t st, int ntab, const BlockInfo* btab, const S
   else
     hipLaunchKernelGGL((k_stats<T, true>), dim3(xcd_grid(ntab)), dim3(256), 0, st, loc, pos, vel,
                        id, offset, home, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_stats_combine(hipStream_t st, int32_t nscenes, int32_t ntab,
+                                const BlockInfo* btab, const StatPart* part, psx_stats* out) {
+  hipLaunchKernelGGL(k_stats_combine, dim3(blocks_for(nscenes, 64)), dim3(64), 0, st, nscenes,
+                     ntab, btab, part, out);
   return hipGetLastError();
 }
 

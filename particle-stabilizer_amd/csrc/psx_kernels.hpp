@@ -1,6 +1,7 @@
 // psx_kernels.hpp -- launchers of the per-step kernels (definitions in psx_kernels.hip).
 #pragma once
 
+#include "psx.h"
 #include "psx_device.hpp"
 
 namespace psx {
@@ -163,6 +164,12 @@ hipError_t launch_stats(hipStream_t st, int ntab, const BlockInfo* btab, const S
                         const SceneDev<T>* one, const Vec4<T>* pos, const Vec4<T>* vel,
                         const int32_t* id, const uint32_t* offset, const uint32_t* home,
                         StatPart* part);
+
+// Per-scene statistics out[0 .. nscenes) from the partials part[0 .. ntab) of a launch_stats over
+// the table btab (one entry per workgroup of 256 particles, scenes in ascending order), combined on
+// the device in the order and with the operations of the host's combination.
+hipError_t launch_stats_combine(hipStream_t st, int32_t nscenes, int32_t ntab,
+                                const BlockInfo* btab, const StatPart* part, psx_stats* out);
 
 template <class T>
 hipError_t launch_unsort(hipStream_t st, int32_t n, const Vec4<T>* pos, const Vec4<T>* vel,

@@ -13,6 +13,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -1150,6 +1151,32 @@ struct SimImpl final : SimBase {
     }
   }
 
+  // statistics() without allocating: the partials and per-scene totals go to `scratch`, the totals
+  // are combined on the device in the host combination's order and copied to out.
+  void statistics_into(psx_stats* out, void* scratch_dev, size_t bytes) override {
+    int nb;
+    const BlockInfo* bt = table(kStatBlock, &nb);  // built at set-up: no allocation here
+    const size_t parts = ((size_t)nb * sizeof(StatPart) + 255) / 256 * 256;
+    const size_t need = parts + scenes.size() * sizeof(psx_stats);
+    if (need > bytes)
+      throw Error(PSX_ERR_INVALID, "psx_statistics_noalloc: " + std::to_string(n) + " particles in " +
+                                       std::to_string(scenes.size()) + " scenes need " +
+                                       std::to_string(need) + " bytes of scratch, more than its " +
+                                       std::to_string(bytes));
+    StatPart* part = static_cast<StatPart*>(scratch_dev);
+    psx_stats* tot = reinterpret_cast<psx_stats*>(static_cast<char*>(scratch_dev) + parts);
+    drop_hash();
+    bucket_and_sort(0, nullptr);
+    PSX_HIP(launch_stats<T>(stream, nb, single() ? nullptr : bt, scenes_d.as<SceneDev<T>>(),
+                            single() ? scenes_h.data() : nullptr, pos[1 - cur_pos].as<Vec4<T>>(),
+                            vel[1 - cur_vel].as<Vec4<T>>(), id[1 - cur_pos].as<int32_t>(),
+                            offset.as<uint32_t>(), home.as<uint32_t>(), part));
+    PSX_HIP(launch_stats_combine(stream, (int32_t)scenes.size(), nb, bt, part, tot));
+    PSX_HIP(hipMemcpyAsync(out, tot, scenes.size() * sizeof(psx_stats), hipMemcpyDeviceToHost,
+                           stream));
+    PSX_HIP(hipStreamSynchronize(stream));
+  }
+
   uint32_t read_flags() override {
     uint32_t f[4];
     PSX_HIP(hipMemcpyAsync(f, flags.p, 16, hipMemcpyDeviceToHost, stream));
@@ -1213,6 +1240,37 @@ using psx::set_error;
 struct psx_sim {
   std::unique_ptr<psx::SimBase> impl;
 };
+
+// Scratch memory of psx_statistics_noalloc: the per-workgroup partials (72 bytes per 256
+// particles) and the per-scene totals, in a fixed device array that every GPU running the library
+// holds until the process ends (about 14 million particles per call at most). Calls that overlap
+// on one GPU share it and race.
+static constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on the current device. The array's name is a host-side stand-in
+// and never a device address: the address is looked up once per device, with that device current,
+// and used for every kernel argument and copy.
+static constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kScratchDevices] = {};
+
+static void* scratch_address() {
+  int dev = -1;
+  PSX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    throw Error(PSX_ERR_INVALID, "psx_statistics_noalloc: device index " + std::to_string(dev) +
+                                     " beyond the " + std::to_string(kScratchDevices) + " supported");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(PSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
 
 #define PSX_TRY(...)                                                          \
   try {                                                                       \
@@ -1517,6 +1575,15 @@ int psx_statistics(psx_sim* sim, psx_stats* out) {
   if (!sim || !out) return set_error(PSX_ERR_INVALID, "NULL argument");
   PSX_TRY({
     sim->impl->statistics(out);
+    return PSX_OK;
+  })
+}
+
+int psx_statistics_noalloc(psx_sim* sim, psx_stats* out) {
+  if (!sim || !out) return set_error(PSX_ERR_INVALID, "NULL argument");
+  PSX_TRY({
+    void* p = scratch_address();
+    sim->impl->statistics_into(out, p, kScratchBytes);
     return PSX_OK;
   })
 }

@@ -75,6 +75,7 @@ _SIGNATURES = {
                                        C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "psx_write_obj_spheres": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "psx_statistics": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "psx_statistics_noalloc": (C.c_int, [C.c_void_p, C.c_void_p]),
     "psx_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(psx_tuning)]),
     "psx_get_state": (C.c_int, [C.c_void_p, _DP, _DP]),
     "psx_set_state": (C.c_int, [C.c_void_p, _DP, _DP]),

@@ -249,6 +249,15 @@ class Simulation:
                  "max": np.array(o.max[:]), "n_collided": int(o.n_collided),
                  "max_overlap": o.max_overlap} for o in out]
 
+    def statistics_noalloc(self) -> list[dict]:
+        """statistics() through psx_statistics_noalloc (no device allocation; fixed scratch)."""
+        from ._lib import psx_stats
+        out = (psx_stats * self.num_scenes)()
+        check(self._lib.psx_statistics_noalloc(self._h, C.cast(out, C.c_void_p)))
+        return [{"kinetic_energy": o.kinetic_energy, "min": np.array(o.min[:]),
+                 "max": np.array(o.max[:]), "n_collided": int(o.n_collided),
+                 "max_overlap": o.max_overlap} for o in out]
+
     def profile(self, nsteps: int) -> dict:
         """Mean device time (ms) per launch of each kernel over nsteps steps (HIP events)."""
         ms = np.zeros(32)
