@@ -80,7 +80,9 @@ typedef struct psx_tuning {
   int32_t use_graph;        /* 1: replay steps from a captured hipGraph */
   int32_t pair_variant;     /* pair-contact kernel: lanes per particle 0 or 1, 2, 4, 8, 16; or 32: one
                                lane per particle with the workgroup's neighbour particles staged in
-                               LDS (block_pairs up to 512, 256 in binary64) */
+                               LDS (block_pairs up to 512, 256 in binary64); or 64: one lane per
+                               particle with each wave's neighbour columns staged in LDS
+                               (block_pairs up to 512 in binary64) */
   int32_t sort_variant;     /* cell lists of the bucketing: 1 per-cell id slots (three kernels per
                                step in a run: scan, rank/gather, pairs), 2 dense lists filled by a
                                scatter kernel (four), 0 automatic (id slots) */

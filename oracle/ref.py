@@ -14,7 +14,8 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "_build" / "libpsx_oracle.so"
+# $PSX_ORACLE_LIBRARY selects another build of the same source (the sanitizer build)
+LIB = Path(os.environ.get("PSX_ORACLE_LIBRARY") or HERE / "_build" / "libpsx_oracle.so")
 
 _lib = None
 _DP = C.POINTER(C.c_double)

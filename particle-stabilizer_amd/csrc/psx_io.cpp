@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <limits>
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
@@ -32,10 +33,14 @@ std::vector<std::string> split_tokens(const std::string& s, char delim) {
 }
 
 // ParticleData::format(double, nChars) (ParticleIO.cpp:39-56): fixed notation with
-// nChars - len(str(int(|v|))) - [v < 0] - 1 decimals (at least 0).
+// nChars - len(str(int(|v|))) - [v < 0] - 1 decimals (at least 0). The reference's (int) cast of
+// |v| is evaluated as its x86-64 build does (cvttsd2si): NaN and |v| >= 2^31 give INT_MIN, whose 11
+// characters leave no decimals, as any |v| >= 2^31 would.
 std::string format_value(double value, int nchars) {
   std::ostringstream oss;
-  int int_len = (int)std::to_string(static_cast<int>(std::abs(value))).length();
+  const double a = std::abs(value);
+  const int iv = (a < 2147483648.0) ? static_cast<int>(a) : std::numeric_limits<int>::min();
+  int int_len = (int)std::to_string(iv).length();
   if (value < 0) int_len += 1;
   int dec = nchars - int_len - 1;
   if (dec < 0) dec = 0;

@@ -1512,6 +1512,126 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
   return hipGetLastError();
 }
 
+// The one-lane pair kernel with each wave's neighbour columns staged in LDS. A wave holds 64
+// consecutive spheres of the cell order; those of one home column (x, y) find all their candidates
+// in column (x + dx, y + dy) inside one contiguous run of the sorted arrays, from the cell below the
+// lowest home cell to the cell above the highest. For each of the nine column offsets the wave
+// loads that run once, with coalesced loads (a lane per entry, at most kWaveWin entries), stores it
+// in its LDS window and walks every lane's own slice from there: each candidate is fetched once per
+// wave instead of once per lane that tests it, which is what bounds the per-lane kernel (its vector
+// memory path is busy throughout). A wave spanning two home columns stages one run for each; a
+// candidate outside its lane's window (a third home column, a home cell moved by the container, a run
+// longer than the window) is read from memory. The candidates, their order and the arithmetic are
+// those of k_pairs, so the results are bit-identical.
+constexpr int kWaveWin = 128;
+
+// Stages entries [ws, ws + wn) of the sorted positions into win[0, wn) (wn <= kWaveWin).
+template <class T>
+__device__ __forceinline__ void wave_stage(const Vec4<T>* __restrict__ pos, uint32_t ws, uint32_t wn,
+                                           Vec4<T>* win, int lane) {
+  if ((uint32_t)lane < wn) win[lane] = pos[ws + (uint32_t)lane];
+  if ((uint32_t)lane + 64u < wn) win[lane + 64] = pos[ws + 64u + (uint32_t)lane];
+}
+
+// The run of column (hx + dx, hy + dy) (column offset d) holding the slices of home cells
+// (hx, hy, za .. zb): [ws, ws + wn) of the sorted arrays, wn clamped to the window; wn = 0 outside
+// the grid. Every argument is uniform over the wave.
+__device__ __forceinline__ void wave_run(const GridDev& g, const uint32_t* __restrict__ offset,
+                                         int32_t hx, int32_t hy, int32_t za, int32_t zb, int d,
+                                         uint32_t& ws, uint32_t& wn) {
+  const int32_t bx = hx - 1 + d / 3, by = hy - 1 + d % 3;
+  ws = 0;
+  wn = 0;
+  if (bx < 0 || bx >= g.n[0] || by < 0 || by >= g.n[1]) return;
+  const int32_t zl = max(min(za, zb) - 1, 0), zh = min(max(za, zb) + 1, g.n[2] - 1);
+  const int32_t cb = g.cell_base + (bx * g.n[1] + by) * g.n[2];
+  ws = offset[cb + zl];
+  wn = min(offset[cb + zh + 1] - ws, (uint32_t)kWaveWin);
+}
+
+template <class T, bool kTable>
+__global__ void __launch_bounds__(1024)
+    k_pairs_wave(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
+                 const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
+                 const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout,
+                 const NextBucket next) {
+  extern __shared__ __align__(16) unsigned char wave_raw[];
+  __shared__ WgTiles wt;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    const int lane = lane_id();
+    Vec4<T>* const win =
+        reinterpret_cast<Vec4<T>*>(wave_raw) + (size_t)(threadIdx.x >> 6) * (2 * kWaveWin);
+    const bool tiles = next.count && next.tiles.tc;
+    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    const bool active = p < p1;  // the active lanes are a prefix of the wave
+    const GridDev& g = sc.grid;
+    int32_t c = -1;
+    const uint64_t act = __ballot(active);
+    if (act != 0) {
+      PairSelf<T> me;
+      uint32_t lo[9], hi[9];
+      int32_t hx = 0, hy = 0, hz = 0, col = -1;
+#pragma unroll
+      for (int d = 0; d < 9; ++d) lo[d] = hi[d] = 0;
+      if (active) {
+        pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+        cell_coords(g, (int32_t)(home[p] & ~kHomeCull), hx, hy, hz);
+        col = hx * g.n[1] + hy;
+      }
+      // the wave's first home column (lane 0's) and the first other one; the first and last lane
+      // of each give the z range of its run
+      const int32_t c0 = __builtin_amdgcn_readfirstlane(col);
+      const uint64_t in0 = __ballot(active && col == c0);
+      const uint64_t rest = __ballot(active && col != c0);
+      const int f1 = rest ? __ffsll((unsigned long long)rest) - 1 : 0;
+      const int32_t c1 = __builtin_amdgcn_readlane(col, f1);
+      const uint64_t in1 = rest ? __ballot(active && col == c1) : 0ull;
+      const int l0 = 63 - __clzll((long long)in0);
+      const int l1 = in1 ? 63 - __clzll((long long)in1) : 0;
+      const int32_t hx0 = __builtin_amdgcn_readfirstlane(hx), hy0 = __builtin_amdgcn_readfirstlane(hy);
+      const int32_t z0a = __builtin_amdgcn_readfirstlane(hz), z0b = __builtin_amdgcn_readlane(hz, l0);
+      const int32_t hx1 = __builtin_amdgcn_readlane(hx, f1), hy1 = __builtin_amdgcn_readlane(hy, f1);
+      const int32_t z1a = __builtin_amdgcn_readlane(hz, f1), z1b = __builtin_amdgcn_readlane(hz, l1);
+      const int grp = (col == c0) ? 0 : ((in1 != 0ull && col == c1) ? 1 : 2);
+      T ax = me.v.x, ay = me.v.y, az = me.v.z;
+#pragma unroll
+      for (int d = 0; d < 9; ++d) {
+        uint32_t ws0, wn0, ws1 = 0, wn1 = 0;
+        wave_run(g, offset, hx0, hy0, z0a, z0b, d, ws0, wn0);
+        if (in1 != 0ull) wave_run(g, offset, hx1, hy1, z1a, z1b, d, ws1, wn1);
+        // every lane stores its entries, then reads those of the others: LDS operations of one
+        // wave take effect in issue order, and the wait keeps the compiler from moving reads up
+        wave_stage(pos, ws0, wn0, win, lane);
+        if (in1 != 0ull) wave_stage(pos, ws1, wn1, win + kWaveWin, lane);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t ws = grp == 1 ? ws1 : ws0;
+        const uint32_t wn = grp == 0 ? wn0 : (grp == 1 ? wn1 : 0u);
+        const Vec4<T>* wb = win + (grp == 1 ? kWaveWin : 0);
+        const uint32_t e = hi[d];
+        for (uint32_t q = lo[d]; q < e; ++q) {
+          const uint32_t k = q - ws;
+          Vec4<T> cj;
+          if (k < wn) cj = wb[k];
+          else cj = pos[q];
+          if (q != (uint32_t)p && pair_near(me, cj)) pair_contact(me, cj, vsn, q, ax, ay, az);
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (active) {
+        const Vec4<T> vn{ax, ay, az, me.v.w};
+        vout[p] = vn;
+        if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+      }
+    }
+    if (tiles) {
+      wt.add(next.tiles, c);
+      wt.end(next.tiles);
+    }
+  });
+}
+
 template <class T, bool kTable>
 static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int block, int lanes,
                                  int chunk, const Vec4<T>* pos, const Vec4<T>* vsn,
@@ -1537,6 +1657,11 @@ static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int bloc
     case 16:
       PSX_LAUNCH((k_pairs_lanes<T, 16, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;
+    case kPairWave:
+      PSX_LAUNCH((k_pairs_wave<T, kTable>), grid, blk,
+                 (size_t)(block / 64) * 2 * kWaveWin * sizeof(Vec4<T>), st, loc, pos, vsn, offset,
+                 home, vout, next);
+      break;
     case kPairStaged:
       PSX_LAUNCH((k_pairs_staged<T, kTable>), grid, blk,
                  (size_t)kStageFactor * (size_t)block * sizeof(Vec4<T>), st, loc, pos, vsn, offset,
@@ -1554,7 +1679,8 @@ hipError_t launch_pairs(hipStream_t st, int ntab, int block, int lanes, int chun
                         const Vec4<T>* pos, const Vec4<T>* vsn, const uint32_t* offset,
                         const uint32_t* home, Vec4<T>* vout, const NextBucket& next) {
   const Locator<T> loc =
-      make_locator(ntab, lanes == kPairStaged ? block : block / lanes, btab, scenes, one);
+      make_locator(ntab, (lanes == kPairStaged || lanes == kPairWave) ? block : block / lanes, btab,
+                   scenes, one);
   if (one)
     return launch_pairs_t<T, false>(st, loc, block, lanes, chunk, pos, vsn, offset, home, vout, next);
   return launch_pairs_t<T, true>(st, loc, block, lanes, chunk, pos, vsn, offset, home, vout, next);

@@ -137,6 +137,9 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
 // The pair kernel form that stages the workgroup's neighbour particles in LDS (pair_variant value;
 // one lane per particle): at most kStageFactor sorted positions per workgroup thread.
 constexpr int kPairStaged = 32;
+// The one-lane pair kernel form that stages each wave's neighbour columns in LDS (pair_variant
+// value): two windows of kWaveWin sorted positions per wave.
+constexpr int kPairWave = 64;
 constexpr int kStageFactor = 12;
 
 // lanes: lanes per particle (1, 2, 4, 8 or 16), or kPairStaged; a workgroup holds block / lanes

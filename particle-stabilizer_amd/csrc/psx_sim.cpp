@@ -623,7 +623,9 @@ struct SimImpl final : SimBase {
   // (one lane per particle, neighbour particles staged in LDS).
   int pair_lanes() const { return tuning.pair_variant > 1 ? tuning.pair_variant : 1; }
   int pair_particles_per_block() const {
-    return pair_lanes() == kPairStaged ? tuning.block_pairs : tuning.block_pairs / pair_lanes();
+    return (pair_lanes() == kPairStaged || pair_lanes() == kPairWave)
+               ? tuning.block_pairs
+               : tuning.block_pairs / pair_lanes();
   }
 
   // Arguments of a hash pass done inside another kernel, for the particles with ids `ids` (in the
@@ -1449,9 +1451,11 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
   if (c.sort_variant < 0 || c.sort_variant > 2)
     return set_error(PSX_ERR_INVALID, "sort_variant must be 0 (automatic), 1 (id blocks) or 2 (dense lists)");
   if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
-      c.pair_variant != 8 && c.pair_variant != 16 && c.pair_variant != psx::kPairStaged)
+      c.pair_variant != 8 && c.pair_variant != 16 && c.pair_variant != psx::kPairStaged &&
+      c.pair_variant != psx::kPairWave)
     return set_error(PSX_ERR_INVALID,
-                     "pair_variant must be 0, 1, 2, 4, 8 or 16 (lanes per particle) or 32 (LDS-staged)");
+                     "pair_variant must be 0, 1, 2, 4, 8 or 16 (lanes per particle), 32 (LDS-staged "
+                     "per workgroup) or 64 (LDS-staged per wave)");
   if (c.pair_chunk != 0 && c.pair_chunk != 2 && c.pair_chunk != 4)
     return set_error(PSX_ERR_INVALID, "pair_chunk must be 0 or 4, or 2");
   // the staged form takes kStageFactor positions per thread of LDS: at most 96 KiB
@@ -1460,6 +1464,9 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
     return set_error(PSX_ERR_INVALID, vec_bytes == 32
                                           ? "the LDS-staged pair kernel takes block_pairs up to 256 in binary64"
                                           : "the LDS-staged pair kernel takes block_pairs up to 512");
+  // the wave-staged form takes two windows of 128 positions per wave of LDS: at most 64 KiB
+  if (c.pair_variant == psx::kPairWave && (c.block_pairs / 64) * 2 * 128 * vec_bytes > 64 * 1024)
+    return set_error(PSX_ERR_INVALID, "the wave-staged pair kernel takes block_pairs up to 512 in binary64");
   PSX_TRY({
     sim->impl->retune(c);
     return PSX_OK;

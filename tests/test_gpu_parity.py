@@ -107,7 +107,9 @@ VARIANTS = [dict(block_pairs=256, block_integrate=256),
             dict(block_pairs=256, block_sort=128, pair_variant=32, sort_variant=1),
             dict(block_pairs=128, pair_variant=1, pair_chunk=2, sort_variant=1),
             dict(block_pairs=1024, pair_chunk=2, sort_variant=2),
-            dict(block_pairs=64, block_integrate=512, pair_variant=32, sort_variant=2)]
+            dict(block_pairs=64, block_integrate=512, pair_variant=32, sort_variant=2),
+            dict(block_pairs=256, pair_variant=64, sort_variant=1),
+            dict(block_pairs=512, block_sort=512, pair_variant=64, sort_variant=2)]
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -188,11 +190,11 @@ def test_mesh_steps_bitexact(psx_mod, oracle, dtype, steps):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, "chunk2"])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, 64, "chunk2"])
 def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
-    """Multi-lane pair kernel (and the LDS-staged one, pair_variant 32, and the one-lane one with
-    two candidates per round) on the mesh scene, eager and graph-replayed; container pushes move
-    some homes out of their workgroup's staged range."""
+    """Multi-lane pair kernel (and the LDS-staged ones, pair_variant 32 per workgroup and 64 per
+    wave, and the one-lane one with two candidates per round) on the mesh scene, eager and
+    graph-replayed; container pushes move some homes out of their staged ranges."""
     sc = small_cylinder(dtype=dtype)
     ref_p, ref_v = oracle_run(oracle, sc, 40)
     shape = dict(pair_variant=1, pair_chunk=2) if lanes == "chunk2" else dict(pair_variant=lanes)
@@ -205,9 +207,11 @@ def test_mesh_pair_lanes_bitexact(psx_mod, oracle, dtype, lanes):
 def test_tuning_rejects_bad_lanes(psx_mod):
     sc = dense_box(dtype="f64")
     with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
-        for bad in (3, 64, -1):
+        for bad in (3, 48, 128, -1):
             with pytest.raises(psx_mod.PsxError):
                 s.set_tuning(pair_variant=bad)
+        with pytest.raises(psx_mod.PsxError, match="512"):
+            s.set_tuning(pair_variant=64, block_pairs=1024)  # binary64: two windows per wave
         for bad in (1, 3, 8):
             with pytest.raises(psx_mod.PsxError, match="pair_chunk"):
                 s.set_tuning(pair_chunk=bad)
