@@ -282,9 +282,10 @@ def bench_batched(steps: int = 128, warmup: int = 10) -> dict:
             "speedup_vs_graphs_stream_each": round(t_graph / tb, 2), "tuning": tuning}
 
 
-def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
+def bench_adjoint(nsteps: int = 2000, every: int = 100, graph: bool = True) -> dict:
     """16,384 spheres in an 80-triangle box mesh, float32: forward window with checkpoints every
-    100 steps against the backward (segment recompute + adjoint)."""
+    100 steps against the backward (segment recompute + adjoint); graph: replay the steps and
+    the backward's segments from hipGraphs (else every kernel is launched on its own)."""
     import torch
 
     import psx
@@ -293,6 +294,7 @@ def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
     n = sc.radius.shape[0]
     sim = psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
     tuned(sim)
+    sim.set_tuning(use_graph=1 if graph else 0)
     dev = torch.device("cuda")
     gx = torch.randn((n, 3), device=dev, dtype=torch.float32)
     gv = torch.randn((n, 3), device=dev, dtype=torch.float32)
@@ -320,7 +322,7 @@ def bench_adjoint(nsteps: int = 2000, every: int = 100) -> dict:
             "ratio": round(tbk / tf, 3), "recompute_ms": round(st["replay_ms"], 2),
             "adjoint_ms_per_step": round(st["adjoint_ms"] / nsteps, 5),
             "forward_ms_per_step": round(1e3 * tf / nsteps, 5),
-            "grad_finite": bool(np.isfinite(g).all())}
+            "graph": graph, "grad_finite": bool(np.isfinite(g).all())}
 
 
 def _plain_summary(r: dict, steps: int) -> dict:
@@ -354,6 +356,8 @@ def main() -> int:
     ap.add_argument("--extras", default="cylinder_settle,batched,adjoint",
                     help="comma-separated extras to run (cylinder_settle, hopper_settle, "
                          "batched, adjoint)")
+    ap.add_argument("--adjoint-graph", type=int, default=1, choices=[0, 1],
+                    help="1: the adjoint extra replays hipGraphs; 0: it launches every kernel")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="finish with one plain device-to-device copy of this many MiB (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -398,7 +402,7 @@ def main() -> int:
                 elif name == "batched":
                     extras[name] = bench_batched()
                 elif name == "adjoint":
-                    extras[name] = bench_adjoint()
+                    extras[name] = bench_adjoint(graph=bool(args.adjoint_graph))
                 else:
                     extras[name] = {"error": "unknown extra"}
             except Exception as e:  # report, keep the headline line
