@@ -1035,6 +1035,123 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
   }
 }
 
+// The binary32 two-wide walk without the nine slices' bounds held in registers. The sphere's walk is
+// summarised in a 27-bit code (bit d: column d is walked; 9 + d: its slice starts at cell z - 1,
+// else at z; 18 + d: it ends at cell z + 1, else at z), from the same culling rule and the same
+// clamping as pair_setup, so the slices are identical. Column d's bounds come from one 16-byte load
+// of the offsets of its cells z - 1 .. z + 2 (the offset arrays carry a spare entry on each side),
+// issued two slices ahead; each slice's first chunk is loaded one slice ahead as in pair_sum. Half
+// the offset loads, and 18 registers fewer for the bounds: measured on MI355X, the 4.2M-sphere
+// hopper's pair kernel 147.5 -> 140.2 us, bit-identical.
+struct __attribute__((aligned(4))) Off4 {
+  uint32_t a, b, c, d;
+};
+
+template <class T>
+__device__ __forceinline__ uint32_t walk_code(const SceneDev<T>& s, const Vec4<T>& x, uint32_t hw,
+                                              int32_t hx, int32_t hy, int32_t hz) {
+  const GridDev& g = s.grid;
+  uint32_t code = 0;
+  if (hw & kHomeCull) {  // as in pair_setup: all 27 neighbours exist and are bounded
+    const double ux = ((double)x.x - g.min[0]) - (double)hx * g.h;
+    const double uy = ((double)x.y - g.min[1]) - (double)hy * g.h;
+    const double uz = ((double)x.z - g.min[2]) - (double)hz * g.h;
+    const double dlo[3] = {fmax(ux, 0.0), fmax(uy, 0.0), fmax(uz, 0.0)};
+    const double dhi[3] = {fmax(g.h - ux, 0.0), fmax(g.h - uy, 0.0), fmax(g.h - uz, 0.0)};
+    const double reach = ((double)x.w + (double)s.rmax) * (1.0 + 1e-4) + 1e-4 * g.h;
+    const double r2 = reach * reach;
+    const double zm = dlo[2] * dlo[2], zp = dhi[2] * dhi[2];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const int dx = d / 3 - 1, dy = d % 3 - 1;
+      const double ex = dx < 0 ? dlo[0] : (dx > 0 ? dhi[0] : 0.0);
+      const double ey = dy < 0 ? dlo[1] : (dy > 0 ? dhi[1] : 0.0);
+      const double exy = ex * ex + ey * ey;
+      if (exy <= r2) {
+        code |= 1u << d;
+        if (exy + zm <= r2) code |= 1u << (9 + d);
+        if (exy + zp <= r2) code |= 1u << (18 + d);
+      }
+    }
+    return code;
+  }
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {  // the grid's clamps
+    const int32_t bx = hx - 1 + d / 3, by = hy - 1 + d % 3;
+    if (bx >= 0 && bx < g.n[0] && by >= 0 && by < g.n[1]) {
+      code |= 1u << d;
+      if (hz > 0) code |= 1u << (9 + d);
+      if (hz < g.n[2] - 1) code |= 1u << (18 + d);
+    }
+  }
+  return code;
+}
+
+// Offsets of cells z - 1 .. z + 2 of column d around home cell `home` (zeros for a column not walked).
+template <int d>
+__device__ __forceinline__ Off4 walk_offsets(const uint32_t* __restrict__ offset, int32_t home,
+                                             int32_t ny, int32_t nz, uint32_t code) {
+  Off4 o{0u, 0u, 0u, 0u};
+  if (code & (1u << d))
+    o = *reinterpret_cast<const Off4*>(offset + home + ((d / 3 - 1) * ny + (d % 3 - 1)) * nz - 1);
+  return o;
+}
+
+template <int d>
+__device__ __forceinline__ void walk_bounds(const Off4& o, uint32_t code, uint32_t& lo,
+                                            uint32_t& hi) {
+  lo = (code & (1u << (9 + d))) ? o.a : o.b;
+  hi = (code & (1u << (18 + d))) ? o.d : o.c;
+  if (!(code & (1u << d))) lo = hi = 0u;
+}
+
+// Slice d: its first chunk (buf, loaded one slice earlier), then the rest; meanwhile slice d + 1's
+// bounds are decoded and its first chunk loaded, and slice d + 2's offsets fetched.
+template <int d, class T>
+__device__ __forceinline__ void walk_slice(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                           const Vec4<T>* __restrict__ vsn,
+                                           const uint32_t* __restrict__ offset, int32_t home,
+                                           int32_t ny, int32_t nz, uint32_t code, uint32_t self,
+                                           uint32_t& lo, uint32_t& hi, Vec4<T> (&buf)[2], Off4& o2,
+                                           T& ax, T& ay, T& az) {
+  const Vec4<T> c0 = buf[0], c1 = buf[1];
+  const uint32_t l = lo, e = hi;
+  if constexpr (d + 1 < 9) {
+    walk_bounds<d + 1>(o2, code, lo, hi);
+    buf[0] = pos[lo];
+    buf[1] = pos[lo + 1];
+  }
+  if constexpr (d + 2 < 9) o2 = walk_offsets<d + 2>(offset, home, ny, nz, code);
+  pair_test2(me, c0, c1, vsn, l, e, self, ax, ay, az);
+  for (uint32_t q = l + 2; q < e; q += 2) {
+    const Vec4<T>* __restrict__ src = pos + q;
+    pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void pair_sum_coded(const SceneDev<T>& sc, const PairSelf<T>& me,
+                                               const Vec4<T>* __restrict__ pos,
+                                               const Vec4<T>* __restrict__ vsn,
+                                               const uint32_t* __restrict__ offset, int32_t home,
+                                               uint32_t code, uint32_t self, T& ax, T& ay, T& az) {
+  const int32_t ny = sc.grid.n[1], nz = sc.grid.n[2];
+  uint32_t lo, hi;
+  Off4 o2 = walk_offsets<0>(offset, home, ny, nz, code);
+  walk_bounds<0>(o2, code, lo, hi);
+  o2 = walk_offsets<1>(offset, home, ny, nz, code);
+  Vec4<T> buf[2] = {pos[lo], pos[lo + 1]};
+  walk_slice<0>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<1>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<2>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<3>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<4>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<5>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<6>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<7>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<8>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+}
+
 // The binary32 two-wide form is held to eight waves per SIMD (at most 64 VGPRs; it needs 67
 // unbounded, and spills nothing bounded): it runs on the workloads that fill the GPU many times
 // over, where more resident waves hide more of each wave's chain of slice loads (1 % on the
@@ -1052,10 +1169,31 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
     int32_t c = -1;
     if (p < p1) {
       PairSelf<T> me;
-      uint32_t lo[9], hi[9];
-      pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
-      T ax = me.v.x, ay = me.v.y, az = me.v.z;
-      pair_sum<T, W>(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
+      T ax, ay, az;
+      if constexpr (W == 2 && sizeof(T) == 4) {
+        me.x = pos[p];
+        me.v = vsn[p];
+        const uint32_t hw = home[p];
+        const int32_t hc = (int32_t)(hw & ~kHomeCull);
+        const T rr = me.x.w + sc.rmax;
+        me.thr = (rr * rr) * PairMargin<T>::value;
+        me.ks = sc.spring;
+        me.es = sc.es;
+        int32_t hx, hy, hz;
+        cell_coords(sc.grid, hc, hx, hy, hz);
+        const uint32_t code = walk_code(sc, me.x, hw, hx, hy, hz);
+        ax = me.v.x;
+        ay = me.v.y;
+        az = me.v.z;
+        pair_sum_coded(sc, me, pos, vsn, offset, hc, code, (uint32_t)p, ax, ay, az);
+      } else {
+        uint32_t lo[9], hi[9];
+        pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+        ax = me.v.x;
+        ay = me.v.y;
+        az = me.v.z;
+        pair_sum<T, W>(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
+      }
       const Vec4<T> vn{ax, ay, az, me.v.w};
       vout[p] = vn;
       if (next.count) c = next_bucket(sc, next, me.x, vn, p);

@@ -39,6 +39,13 @@ constexpr int kListPad = 4;
 // loads may reach three entries past the last particle).
 constexpr int kPairPad = 4;
 
+// Spare entries around every cell-offset array the pair kernel reads: the binary32 two-wide walk
+// loads the offsets of cells z - 1 .. z + 2 of a column with one 16-byte load, so it reads one entry
+// before the first cell's and one past the final total. Arrays are allocated with kOffsetPad spare
+// entries and passed to the kernels kOffsetLead entries in.
+constexpr int kOffsetLead = 1;
+constexpr int kOffsetPad = 3;
+
 // The next step's hash pass (launch_integrate's work on a cell-sorted state), done by the pair kernel
 // for each particle right after its new velocity: x + v_new dt, the cell of that position, its
 // count and, with id blocks (blocks != 0), its entry in the cell's id block. The state the next

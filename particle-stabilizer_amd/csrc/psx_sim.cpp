@@ -477,7 +477,7 @@ struct SimImpl final : SimBase {
     ovf.alloc((size_t)nn * 8);
     ovf_ctr.alloc(8);
     count.alloc((size_t)cpad * 4);
-    offset.alloc(((size_t)cpad + 1) * 4);
+    offset.alloc(((size_t)cpad + 1 + kOffsetPad) * 4);
     tile_state.alloc((size_t)ntiles * 8);
     tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct * kTileStride, 1) * 4);
     scan_gen.alloc(4);
@@ -613,11 +613,14 @@ struct SimImpl final : SimBase {
     Vec4<T>* p = pos[a].as<Vec4<T>>();
     Vec4<T>* v = vel[vc].as<Vec4<T>>();
     bucket(p, v, id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(),
-           id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr, mode, prof);
+           id[1 - a].as<int32_t>(), offs(), nullptr, mode, prof);
   }
 
   // A single scene launches without the workgroup table (constants by value, slices by index).
   bool single() const { return scenes_h.size() == 1; }
+
+  // The cell offsets (C_pad + 1 entries) inside their buffer, which holds spare entries around them.
+  uint32_t* offs() const { return offset.as<uint32_t>() + kOffsetLead; }
 
   // Form of the pair kernel (tuning.pair_variant; 0 means 1): lanes per particle, or kPairStaged
   // (one lane per particle, neighbour particles staged in LDS).
@@ -689,9 +692,9 @@ struct SimImpl final : SimBase {
     const int a = cur_pos, vc = cur_vel;
     ensure_hashed(prof);
     order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
-          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offset.as<uint32_t>(), nullptr,
+          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offs(), nullptr,
           kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr);
-    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offset.as<uint32_t>(),
+    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offs(),
           vel[vc].as<Vec4<T>>(), prof, id[1 - a].as<int32_t>());
     cur_pos = 1 - a;
   }
@@ -754,7 +757,8 @@ struct SimImpl final : SimBase {
       rs_xpre.alloc((size_t)every * nn * sizeof(Vec4<T>));
       rs_id.alloc((size_t)every * nn * 4);
       rs_cell.alloc((size_t)every * nn * 4);
-      rs_off.alloc((size_t)every * ((size_t)cpad + 1) * 4);  // k_scan writes cpad + 1
+      // k_scan writes cpad + 1 per record; spare entries around the records as around `offset`
+      rs_off.alloc(((size_t)every * ((size_t)cpad + 1) + kOffsetPad) * 4);
       for (DevBuf* b : {&lxA, &lvA, &lxB, &lvB}) b->alloc(3 * nn * sizeof(T));
       for (DevBuf* b : {&gk, &ge, &gg}) b->alloc(nn * sizeof(T));
       gout.alloc(3 * sizeof(double) * scenes.size());
@@ -792,7 +796,7 @@ struct SimImpl final : SimBase {
     for (int64_t k = 0; k < ks; ++k) {
       Vec4<T>* sp = rs_pos.as<Vec4<T>>() + k * nn;
       Vec4<T>* sv = rs_vel.as<Vec4<T>>() + k * nn;
-      uint32_t* so = rs_off.as<uint32_t>() + k * nc;
+      uint32_t* so = rs_off.as<uint32_t>() + kOffsetLead + k * nc;
       int32_t* si = rs_id.as<int32_t>() + k * nn;
       order(pre_pos(k), pre_vel(k), pre_id(k), sp, sv, si, so, rs_cell.as<int32_t>() + k * nn,
             kModeIntegrate | kModeContainer | kModeCheck, nullptr, rs_xpre.as<Vec4<T>>() + k * nn);
@@ -820,7 +824,8 @@ struct SimImpl final : SimBase {
             offH.as<uint32_t>(), nullptr, 0, nullptr, nullptr);
       const NextBucket hn = next_bucket_args(k > 0 ? si - nn : nullptr);
       PSX_HIP(launch_adj_step<T>(stream, nb, bt, scenes_d.as<SceneDev<T>>(), sp, sv, si,
-                                 rs_off.as<uint32_t>() + k * nc, rs_cell.as<int32_t>() + k * nn,
+                                 rs_off.as<uint32_t>() + kOffsetLead + k * nc,
+                                 rs_cell.as<int32_t>() + k * nn,
                                  rs_xpre.as<Vec4<T>>() + k * nn, posH.as<Vec4<T>>(),
                                  velH.as<Vec4<T>>(), idH.as<int32_t>(), offH.as<uint32_t>(),
                                  tgeo_d.as<TriGeo<T>>(), tbox_d.as<TriBox<T>>(),
@@ -1098,7 +1103,7 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipMemcpyAsync(hc.data(), cell_of.p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
     PSX_HIP(hipMemcpyAsync(hid.data(), id[a].p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
     PSX_HIP(hipMemcpyAsync(hperm.data(), id[1 - a].p, (size_t)nn * 4, hipMemcpyDeviceToHost, stream));
-    PSX_HIP(hipMemcpyAsync(hoff.data(), offset.p, ((size_t)ncells) * 4, hipMemcpyDeviceToHost, stream));
+    PSX_HIP(hipMemcpyAsync(hoff.data(), offs(), ((size_t)ncells) * 4, hipMemcpyDeviceToHost, stream));
     PSX_HIP(hipStreamSynchronize(stream));
     hoff[(size_t)ncells] = (uint32_t)nn;  // offset[C] == n (padding cells are empty)
     cur_pos = 1 - a;
@@ -1124,7 +1129,7 @@ struct SimImpl final : SimBase {
     PSX_HIP(launch_stats<T>(stream, nb, single() ? nullptr : bt, scenes_d.as<SceneDev<T>>(),
                             single() ? scenes_h.data() : nullptr, pos[1 - cur_pos].as<Vec4<T>>(),
                             vel[1 - cur_vel].as<Vec4<T>>(), id[1 - cur_pos].as<int32_t>(),
-                            offset.as<uint32_t>(), home.as<uint32_t>(), stat_part.as<StatPart>()));
+                            offs(), home.as<uint32_t>(), stat_part.as<StatPart>()));
     std::vector<StatPart> h((size_t)nb);
     PSX_HIP(hipMemcpyAsync(h.data(), stat_part.p, h.size() * sizeof(StatPart), hipMemcpyDeviceToHost,
                            stream));
@@ -1172,7 +1177,7 @@ struct SimImpl final : SimBase {
     PSX_HIP(launch_stats<T>(stream, nb, single() ? nullptr : bt, scenes_d.as<SceneDev<T>>(),
                             single() ? scenes_h.data() : nullptr, pos[1 - cur_pos].as<Vec4<T>>(),
                             vel[1 - cur_vel].as<Vec4<T>>(), id[1 - cur_pos].as<int32_t>(),
-                            offset.as<uint32_t>(), home.as<uint32_t>(), part));
+                            offs(), home.as<uint32_t>(), part));
     PSX_HIP(launch_stats_combine(stream, (int32_t)scenes.size(), nb, bt, part, tot));
     PSX_HIP(hipMemcpyAsync(out, tot, scenes.size() * sizeof(psx_stats), hipMemcpyDeviceToHost,
                            stream));
