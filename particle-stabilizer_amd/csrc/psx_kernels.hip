@@ -652,7 +652,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     uint32_t dst = 0;
     int32_t me = 0;
     int32_t k0 = 0, k1 = 0;  // triangle list of the particle's cell
-    uint8_t cull_c = 0;      // cull flag of the bucket cell
+    uint8_t cull_c = 0;      // flag byte of the bucket cell
     const bool mesh = (mode & kModeContainer) && s.mesh >= 0;
     if (active) {
       c = cell_of[p];
@@ -663,12 +663,12 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
         integrate_position(s, x, v);
         integrate_velocity(s, v);
       }
-      if (mesh) {
+      cull_c = cull[c];
+      if (mesh && (cull_c & kCellTris)) {  // most cells list no triangle: their bounds go unread
         const int32_t tc = s.tcell_base + (c - s.grid.cell_base);
         k0 = tcell_off[tc];
         k1 = tcell_off[tc + 1];
       }
-      cull_c = cull[c];
       const uint32_t b = offset[c], e = offset[c + 1];
       uint32_t rank = 0;
       if constexpr (kBlocks) {
@@ -716,7 +716,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       home = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
       cull_c = cull[home];
     }
-    home_o[dst] = (uint32_t)home | (cull_c ? kHomeCull : 0u);
+    home_o[dst] = (uint32_t)home | ((cull_c & kCellCull) ? kHomeCull : 0u);
     if (xpre_o) xpre_o[dst] = xpre;
     if (mode & kModeCheck) {
       const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&

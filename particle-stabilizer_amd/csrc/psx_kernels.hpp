@@ -23,6 +23,9 @@ constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 constexpr int64_t kIdPlane = 4096;
 constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the cell's overflow chain
 constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours
+// Per-cell flag byte (static per scene): the cell's 27 neighbours may be culled by distance in the
+// pair kernel; the cell's triangle list is not empty (only such cells' lists are read).
+constexpr uint8_t kCellCull = 1, kCellTris = 2;
 // Largest workgroup of the rank/gather kernel: its binary64 form (rank, walls and the mesh contact
 // in double) needs more than the 128 registers a 1024-wide workgroup leaves per lane.
 template <class T>

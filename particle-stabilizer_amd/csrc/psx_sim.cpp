@@ -194,12 +194,14 @@ void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32) {
   m.off[(size_t)ncell] = (int32_t)m.list.size();
 }
 
-// Cells whose 27 neighbours may be culled by distance in the pair kernel (one byte per cell of the
-// scene's grid). A cell is bounded when every member's snapshot position lies inside its box:
-// interior cells (index 1 .. n-2 on every axis; positions outside the domain are clamped into the
-// boundary cells, and box walls only push particles of boundary cells) with an empty triangle list
-// (a particle pushed by the mesh touched a triangle, which its bucket cell therefore lists). The
-// result marks cells whose 3x3x3 neighbourhood is bounded: the bounded mask eroded along x, y, z.
+// The per-cell flag byte of the scene's grid. kCellCull: the cell's 27 neighbours may be culled by
+// distance in the pair kernel. A cell is bounded when every member's snapshot position lies inside
+// its box: interior cells (index 1 .. n-2 on every axis; positions outside the domain are clamped
+// into the boundary cells, and box walls only push particles of boundary cells) with an empty
+// triangle list (a particle pushed by the mesh touched a triangle, which its bucket cell therefore
+// lists). kCellCull marks cells whose 3x3x3 neighbourhood is bounded: the bounded mask eroded along
+// x, y, z. kCellTris: the cell lists at least one triangle (the rank/gather kernel reads the list
+// bounds of those cells only).
 std::vector<uint8_t> cull_cells(const HostGrid& g, const MeshHost* m) {
   const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const size_t nc = (size_t)(nx * ny * nz);
@@ -230,6 +232,9 @@ std::vector<uint8_t> cull_cells(const HostGrid& g, const MeshHost* m) {
   erode(a, b, 0);
   erode(b, a, 1);
   erode(a, b, 2);
+  if (m)
+    for (size_t c = 0; c < nc; ++c)
+      if (m->off[c + 1] != m->off[c]) b[c] |= kCellTris;
   return b;
 }
 
