@@ -11,7 +11,8 @@
  *     kept by runSimulateHIP in psx_compat.hpp; the POD entry points below are what that shim and
  *     every other caller (ctypes, the CLI, the tests) bind.
  *   - runSimulate(...)       include/ParticleStabilizer/PhysicsEngine.hpp:31-42 (CPU backend, same
- *     signature). Its step order (PhysicsEngine.cpp:38-48) is the semantics implemented here.
+ *     signature). Its step order (PhysicsEngine.cpp:38-48) is the semantics implemented here by
+ *     default; PSX_FLAG_SEMANTICS_CUDA selects the reference CUDA backend's step instead.
  *
  * Threading: a psx_sim handle is not thread-safe; separate handles are independent. All device work
  * of a handle is ordered on one HIP stream (its own, or one injected with psx_set_stream).
@@ -26,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PSX_ABI_VERSION 4
+#define PSX_ABI_VERSION 5
 
 enum psx_status {
   PSX_OK = 0,
@@ -53,8 +54,21 @@ typedef struct psx_params {
   double box_min[3];   /* box container; ignored when a mesh is given */
   double box_max[3];
   int32_t dtype;       /* enum psx_dtype: arithmetic type of the particle state on the device */
-  int32_t flags;       /* reserved, must be 0 */
+  int32_t flags;       /* 0 or PSX_FLAG_SEMANTICS_CUDA; other bits must be 0 */
 } psx_params;
+
+/* psx_params.flags: the step of the reference's CUDA backend (PhysicsEngineCUDA.cpp:36-43) instead of
+ * its CPU backend: container before bucketing (membership and home cell from the post-container
+ * position), cell width r_max * 2.0 * 1.05 (ParticleModelCUDA.cpp:84), the pair kick of
+ * ParticleModel_kernels.cu:163-199 (spring term divided by m_i), the mesh contact of
+ * PolygonContainerCUDA_kernels.cu:69-294 (hits d^2 < r^2 from the edge picked by dot(center, q),
+ * position corrections from the incoming position summed, reflections averaged), normals
+ * cross / length (PolygonContainerCUDA.cpp:64-82). The reference adds the kicks and the mesh terms
+ * with atomics in no fixed order; here they are added in one of those orders: cells x-outer, y,
+ * z-inner, members by ascending id, triangles by ascending index. Fixed at creation: every scene of
+ * a batch must carry the same flag, and psx_set_params cannot change it. The differentiable window
+ * implements the CPU semantics only. */
+#define PSX_FLAG_SEMANTICS_CUDA 1
 
 /* A triangle-soup container (PolygonContainer.cpp:8-90): n_triangles de-indexed triangles,
  * vertices[9*t + 3*k + a] = coordinate a of corner k of triangle t. Normals follow the winding,
@@ -192,12 +206,13 @@ int psx_statistics_noalloc(psx_sim* sim, psx_stats* out);
  * contact sets (pair contacts, wall and triangle hits) fixed. */
 
 /* Replaces the coefficients of one scene (gravity, walls, spring, restitution, dt); the container
- * bounds and dtype must be unchanged. */
+ * bounds, dtype and flags must be unchanged. */
 int psx_set_params(psx_sim* sim, int32_t scene, const psx_params* params);
 
 /* Runs nsteps steps from the current state, keeping a copy of the state every checkpoint_every
  * steps in device memory for psx_window_backward. *window_id (may be NULL) receives the window's
- * id; a handle holds the checkpoints of its latest window only. */
+ * id; a handle holds the checkpoints of its latest window only. PSX_ERR_STATE for a handle created
+ * with PSX_FLAG_SEMANTICS_CUDA. */
 int psx_window_forward(psx_sim* sim, int64_t nsteps, int64_t checkpoint_every,
                        int64_t* window_id);
 

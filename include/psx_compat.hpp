@@ -10,7 +10,9 @@
 //
 // Conventions kept from the reference: synchronous, device 0, errors are logged and end the process
 // with exit(EXIT_FAILURE) (CudaCommon.cuh:18-28). New code should call the C ABI in psx.h instead,
-// which never exits.
+// which never exits. The signature has no room for options, so two come from the environment:
+// PSX_DTYPE=f32 (device arithmetic; binary64 otherwise) and PSX_SEMANTICS=cuda (the reference
+// CUDA engine's step, PSX_FLAG_SEMANTICS_CUDA; its CPU engine's step otherwise).
 #pragma once
 
 #include <cstddef>

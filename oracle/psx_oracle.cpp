@@ -30,6 +30,20 @@
 // submodule is empty), so bit-level agreement with a glm build is "parity unpinned" beyond the glm
 // definitions above.
 //
+// The reference's CUDA backend (src/ParticleStabilizerCUDA/*) is restated as well, for the
+// PSX_FLAG_SEMANTICS_CUDA mode of the product (semantics "cuda" here):
+//   step order            PhysicsEngineCUDA.cpp:36-43 (integrate, container, bucket, pairs)
+//   grid geometry         ParticleModelCUDA.cpp:62-99 (width r_max * 2.0 * 1.05, exact 1.05)
+//   pair contact          ParticleModel_kernels.cu:122-199 (spring term / m_i; kicks added onto v)
+//   mesh container        PolygonContainerCUDA.cpp:30-99 (normals cross / length),
+//                         PolygonContainerCUDA_kernels.cu:69-294 (edge point picked by
+//                         dot(center, q), hits |q - x|^2 < r^2, summed corrections from x)
+// The reference adds the pair kicks and the mesh terms with atomics in no fixed order and lets
+// nvcc contract multiply-adds; here they are added in one of those orders (cells x-outer, y,
+// z-inner, members by ascending id, triangles ascending) with every operation rounded on its own.
+// The CUDA count kernel does not clamp a cell index (ParticleModel_kernels.cu:49-56); positions
+// outside the grid are clamped here, as the neighbour search clamps them.
+//
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp, no -march: plain SSE2 double/float
 // arithmetic, as the reference's Release build).
 
@@ -97,11 +111,13 @@ struct Grid {
 };
 
 // ParticleBucket::initBucket, ParticleBucket.cpp:17-27; COEFF_FOR_INTERVAL_OF_BUCKETS = 1.05f stored
-// in a double (ParticleBucket.hpp:17).
-Grid make_grid(double rmax, const double* mn, const double* mx) {
+// in a double (ParticleBucket.hpp:17). cuda: ParticleModelCUDA::initBuckets
+// (ParticleModelCUDA.cpp:62-99), bucketInterval = maxRadius * 2.0 * 1.05 and
+// (int)((max - min) / interval) + 1 cells per axis.
+Grid make_grid(double rmax, const double* mn, const double* mx, bool cuda = false) {
   Grid g;
   const double coeff = (double)1.05f;
-  g.h = rmax * 2.0 * coeff;
+  g.h = cuda ? rmax * 2.0 * 1.05 : rmax * 2.0 * coeff;
   for (int a = 0; a < 3; ++a) {
     g.min[a] = mn[a];
     g.max[a] = mx[a];
@@ -201,8 +217,11 @@ struct Mesh {
 
 // PolygonContainer ctor (PolygonContainer.cpp:52-89) and bounds (288-318). Normals are evaluated in
 // double from the double vertices and then stored in T; AABBs are taken over the T vertices.
+// cuda: PolygonContainerCUDA ctor (PolygonContainerCUDA.cpp:30-99): the same cross product
+// (relVec0 x relVec1 with relVec0 = v1 - v0, relVec1 = v2 - v0), divided component by component
+// by its length; the same AABBs and bounds (getMinCoords / getMaxCoords, :228-262).
 template <class T>
-Mesh<T> make_mesh(int64_t nt, const double* v) {
+Mesh<T> make_mesh(int64_t nt, const double* v, bool cuda = false) {
   Mesh<T> m;
   m.nt = nt;
   m.a.resize(nt);
@@ -214,7 +233,8 @@ Mesh<T> make_mesh(int64_t nt, const double* v) {
   for (int64_t t = 0; t < nt; ++t) {
     const double* p = v + 9 * t;
     V3<double> a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
-    V3<double> nn = normalize(cross(b - a, c - a));
+    const V3<double> cr = cross(b - a, c - a);
+    const V3<double> nn = cuda ? cr / length(cr) : normalize(cr);
     m.a[t] = {(T)a.x, (T)a.y, (T)a.z};
     m.b[t] = {(T)b.x, (T)b.y, (T)b.z};
     m.c[t] = {(T)c.x, (T)c.y, (T)c.z};
@@ -287,6 +307,31 @@ inline V3<T> closest_on_triangle(V3<T> a, V3<T> b, V3<T> c, V3<T> n, V3<T> p) {
   return proj;
 }
 
+// getClosestPointOnPolygonKernelFunc (PolygonContainerCUDA_kernels.cu:125-154): the same
+// projection, inside test and segment points as closest_on_triangle, but the edge point is picked by
+// dot(center, q), not by the distance |q - center|.
+template <class T>
+inline V3<T> closest_on_triangle_cuda(V3<T> a, V3<T> b, V3<T> c, V3<T> n, V3<T> p) {
+  V3<T> proj = project_onto_plane(p, a, n);
+  if (point_on_triangle(proj, a, b, c)) return proj;
+  const V3<T> qab = closest_on_segment(p, a, b);
+  T best = dot(p, qab);
+  proj = qab;
+  const V3<T> qbc = closest_on_segment(p, b, c);
+  const T dbc = dot(p, qbc);
+  if (dbc < best) {
+    best = dbc;
+    proj = qbc;
+  }
+  const V3<T> qca = closest_on_segment(p, c, a);
+  const T dca = dot(p, qca);
+  if (dca < best) {
+    best = dca;
+    proj = qca;
+  }
+  return proj;
+}
+
 template <class T>
 struct Params {
   V3<T> g;
@@ -308,6 +353,7 @@ struct Sim {
   // its own, read from the snapshot, so the results are those of the serial loop)
   bool fast = false;
   int nthreads = 1;
+  bool cuda = false;  // the reference CUDA backend's step
   // scratch
   std::vector<V3<T>> sx, sv, bv, ybuf, accbuf;
   std::vector<int64_t> counter;
@@ -425,6 +471,56 @@ void mesh_contact(Sim<T>& s, int nthreads) {  // PolygonContainer.cpp:237-286
   }
 }
 
+// PolygonContainerCUDA::resolveCollisions (PolygonContainerCUDA.cpp:130-214): isIntersectedKernel
+// (PolygonContainerCUDA_kernels.cu:179-211: AABB overlap with <= / >=, then |q - x|^2 < r^2),
+// calcModifiedVelocitySumKernel and calcAveragedVelocityKernel (:214-252: v = mean of
+// e (v - (2 v.n) n) over the hits, when there is one), calcPosModificationAmountKernel and
+// modifyPositionKernel (:254-286: x += sum of -(r - dot(-n, x - a)) n, for every particle). All terms
+// come from the incoming x and v; the atomics' sums are taken in ascending triangle order.
+template <class T>
+void mesh_contact_cuda(Sim<T>& s, int nthreads) {
+  const Mesh<T>& m = *s.mesh;
+  const T e = s.p.ew_mesh;
+  const int64_t n = s.n;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+  std::vector<int32_t> cand;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<T> x = s.x[i];
+    const V3<T> v = s.v[i];
+    const T r = s.r[i];
+    const V3<T> lo = {x.x - r, x.y - r, x.z - r};
+    const V3<T> hi = {x.x + r, x.y + r, x.z + r};
+    V3<T> vb{0, 0, 0}, mod{0, 0, 0};
+    int64_t cnt = 0;
+    if (m.indexed) m.candidates(lo, hi, cand);
+    const int64_t ncand = m.indexed ? (int64_t)cand.size() : m.nt;
+    for (int64_t k = 0; k < ncand; ++k) {
+      const int64_t t = m.indexed ? (int64_t)cand[(size_t)k] : k;
+      if (!(lo.x <= m.hi[t].x && hi.x >= m.lo[t].x && lo.y <= m.hi[t].y && hi.y >= m.lo[t].y &&
+            lo.z <= m.hi[t].z && hi.z >= m.lo[t].z))
+        continue;
+      const V3<T> q = closest_on_triangle_cuda(m.a[t], m.b[t], m.c[t], m.n[t], x);
+      const V3<T> rel = q - x;
+      if (!(dot(rel, rel) < r * r)) continue;
+      const V3<T> nn = m.n[t];
+      const V3<T> refl = v - (T(2) * dot(v, nn)) * nn;
+      vb = vb + e * refl;
+      const T sc = r - dot(T(-1) * nn, x - m.a[t]);
+      mod = mod + (-sc) * nn;
+      ++cnt;
+    }
+    if (cnt > 0) s.v[i] = vb / (T)cnt;
+    s.x[i] = x + mod;
+  }
+  }
+}
+
 template <class T>
 void pair_contact(Sim<T>& s) {  // ParticleModel.cpp:64-144
   const int64_t n = s.n;
@@ -455,6 +551,22 @@ void pair_contact(Sim<T>& s) {  // ParticleModel.cpp:64-144
             if (j == i) continue;
             const V3<T> xj = s.sx[j];
             const T rj = s.r[j];
+            if (s.cuda) {  // resolveCollisionsKernel, ParticleModel_kernels.cu:163-199
+              const V3<T> d = xi - xj;
+              const T dsq = (d.x * d.x + d.y * d.y) + d.z * d.z;
+              const T dist = std::sqrt(dsq);
+              const T ovc = (ri + rj) - dist;
+              if (!(ovc > T(0))) continue;
+              const V3<T> vj = s.sv[j];
+              const T mj = s.m[j];
+              const V3<T> rv = vi - vj;
+              const T rvd = (rv.x * d.x + rv.y * d.y) + rv.z * d.z;
+              const T crel = (((T(2) * mj) * rvd) * es) / ((mi + mj) * dsq);
+              const T cspr = (k * ovc) / (dist * mi);
+              const T f = cspr - crel;
+              acc = V3<T>{acc.x + f * d.x, acc.y + f * d.y, acc.z + f * d.z};
+              continue;
+            }
             if (!(distance(xi, xj) < ri + rj)) continue;  // isHit
             const V3<T> vj = s.sv[j];
             const T mj = s.m[j];
@@ -474,6 +586,16 @@ void pair_contact(Sim<T>& s) {  // ParticleModel.cpp:64-144
 
 template <class T>
 void step(Sim<T>& s, int nthreads) {  // PhysicsEngine.cpp:38-48
+  if (s.cuda) {  // PhysicsEngineCUDA.cpp:36-43: container before bucketing
+    integrate(s);
+    if (s.mesh)
+      mesh_contact_cuda(s, nthreads);
+    else
+      box_walls(s);
+    update_buckets(s);
+    pair_contact(s);
+    return;
+  }
   integrate(s);
   update_buckets(s);
   if (s.mesh)
@@ -500,7 +622,9 @@ Params<T> make_params(const double* p) {
 
 template <class T>
 int run(const double* params, int64_t n, T* pos, T* vel, const T* r, const T* m, int64_t ntri,
-        const double* tri, int64_t nsteps, int nthreads, int fast) {
+        const double* tri, int64_t nsteps, int nthreads, int opts) {
+  const int fast = opts & 1;
+  const bool cuda = (opts & 2) != 0;
   if (n <= 0 || !params || !pos || !vel || !r || !m) return 1;
   Sim<T> s;
   s.n = n;
@@ -515,10 +639,11 @@ int run(const double* params, int64_t n, T* pos, T* vel, const T* r, const T* m,
   s.p = make_params<T>(params);
   s.fast = fast != 0;
   s.nthreads = std::max(nthreads, 1);
+  s.cuda = cuda;
   Mesh<T> mesh;
   double mn[3], mx[3];
   if (ntri > 0) {
-    mesh = make_mesh<T>(ntri, tri);
+    mesh = make_mesh<T>(ntri, tri, cuda);
     if (fast) mesh.build_index();
     s.mesh = &mesh;
     for (int k = 0; k < 3; ++k) {
@@ -533,7 +658,7 @@ int run(const double* params, int64_t n, T* pos, T* vel, const T* r, const T* m,
   }
   double rmax = 0.0;  // ParticleModel::initBuckets, ParticleModel.cpp:35-38
   for (int64_t i = 0; i < n; ++i) rmax = std::max(rmax, (double)r[i]);
-  s.g = make_grid(rmax, mn, mx);
+  s.g = make_grid(rmax, mn, mx, cuda);
   s.buckets.resize((size_t)(s.g.n[0] * s.g.n[1] * s.g.n[2]));
   for (int64_t it = 0; it < nsteps; ++it) step(s, nthreads);
   for (int64_t i = 0; i < n; ++i) {
@@ -576,9 +701,10 @@ int bucketing(double rmax, const double* mn, const double* mx, int64_t n, const 
 
 extern "C" {
 
-// Grid geometry of ParticleBucket::initBucket: h and cell counts.
-void oracle_grid(double rmax, const double* mn, const double* mx, double* h, int64_t* n3) {
-  Grid g = make_grid(rmax, mn, mx);
+// Grid geometry of ParticleBucket::initBucket (cuda: ParticleModelCUDA::initBuckets): h and cell
+// counts.
+void oracle_grid(double rmax, const double* mn, const double* mx, int cuda, double* h, int64_t* n3) {
+  Grid g = make_grid(rmax, mn, mx, cuda != 0);
   *h = g.h;
   for (int a = 0; a < 3; ++a) n3[a] = g.n[a];
 }
@@ -590,19 +716,20 @@ int64_t oracle_axis_index(double x, double mn, double h, int64_t n) {
 // Full runs. params = gravity[3], walls[6], spring, restitution, dt, box_min[3], box_max[3]
 // (18 doubles). tri: 9*ntri double vertices (may be NULL when ntri == 0). pos/vel are updated in
 // place (n x 3).
-// fast != 0 (long test runs): the mesh loop visits only the triangles a triangle-grid index
-// offers and the pair loop runs on nthreads threads; both give the results of the reference's
-// loops. 0: every triangle and a serial pair loop, as the reference runs them.
+// opts bit 0 (fast, long test runs): the mesh loop visits only the triangles a triangle-grid
+// index offers and the pair loop runs on nthreads threads; both give the results of the
+// reference's loops. Without it: every triangle and a serial pair loop, as the reference runs
+// them. opts bit 1: the reference CUDA backend's step instead of the CPU backend's.
 int oracle_run_f64(const double* params, int64_t n, double* pos, double* vel, const double* r,
                    const double* m, int64_t ntri, const double* tri, int64_t nsteps,
-                   int nthreads, int fast) {
-  return run<double>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, fast);
+                   int nthreads, int opts) {
+  return run<double>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, opts);
 }
 
 int oracle_run_f32(const double* params, int64_t n, float* pos, float* vel, const float* r,
                    const float* m, int64_t ntri, const double* tri, int64_t nsteps, int nthreads,
-                   int fast) {
-  return run<float>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, fast);
+                   int opts) {
+  return run<float>(params, n, pos, vel, r, m, ntri, tri, nsteps, nthreads, opts);
 }
 
 int oracle_bucketing_f64(double rmax, const double* mn, const double* mx, int64_t n,
@@ -624,6 +751,19 @@ void oracle_closest_point_f64(const double* a, const double* b, const double* c,
       P{p[0], p[1], p[2]};
   V3<double> N = normalize(cross(B - A, C - A));
   V3<double> Q = closest_on_triangle(A, B, C, N, P);
+  q[0] = Q.x;
+  q[1] = Q.y;
+  q[2] = Q.z;
+}
+
+// getClosestPointOnPolygonKernelFunc for one triangle (normal cross / length, the CUDA backend's).
+void oracle_closest_point_cuda_f64(const double* a, const double* b, const double* c,
+                                   const double* p, double* q) {
+  V3<double> A{a[0], a[1], a[2]}, B{b[0], b[1], b[2]}, C{c[0], c[1], c[2]},
+      P{p[0], p[1], p[2]};
+  const V3<double> cr = cross(B - A, C - A);
+  V3<double> N = cr / length(cr);
+  V3<double> Q = closest_on_triangle_cuda(A, B, C, N, P);
   q[0] = Q.x;
   q[1] = Q.y;
   q[2] = Q.z;

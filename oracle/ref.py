@@ -35,7 +35,7 @@ def load() -> C.CDLL:
         if not LIB.exists():
             build()
         lib = C.CDLL(str(LIB))
-        lib.oracle_grid.argtypes = [C.c_double, _DP, _DP, _DP, C.POINTER(C.c_int64)]
+        lib.oracle_grid.argtypes = [C.c_double, _DP, _DP, C.c_int, _DP, C.POINTER(C.c_int64)]
         lib.oracle_grid.restype = None
         lib.oracle_axis_index.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int64]
         lib.oracle_axis_index.restype = C.c_int64
@@ -49,6 +49,8 @@ def load() -> C.CDLL:
                                              _U32P, _I32P]
         lib.oracle_closest_point_f64.argtypes = [_DP, _DP, _DP, _DP, _DP]
         lib.oracle_closest_point_f64.restype = None
+        lib.oracle_closest_point_cuda_f64.argtypes = [_DP, _DP, _DP, _DP, _DP]
+        lib.oracle_closest_point_cuda_f64.restype = None
         lib.oracle_mesh_normals_f64.argtypes = [C.c_int64, _DP, _DP]
         lib.oracle_mesh_normals_f64.restype = None
         lib.oracle_mass.argtypes = [C.c_double, C.c_double]
@@ -62,13 +64,14 @@ def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-def grid(rmax: float, mn, mx):
+def grid(rmax: float, mn, mx, semantics: str = "cpu"):
     lib = load()
     mn = np.ascontiguousarray(mn, np.float64)
     mx = np.ascontiguousarray(mx, np.float64)
     h = C.c_double()
     n3 = np.zeros(3, np.int64)
-    lib.oracle_grid(rmax, _p(mn, C.c_double), _p(mx, C.c_double), C.byref(h), _p(n3, C.c_int64))
+    lib.oracle_grid(rmax, _p(mn, C.c_double), _p(mx, C.c_double), _cuda(semantics), C.byref(h),
+                    _p(n3, C.c_int64))
     return h.value, n3
 
 
@@ -76,12 +79,19 @@ def axis_index(x: float, mn: float, h: float, n: int) -> int:
     return int(load().oracle_axis_index(x, mn, h, n))
 
 
+def _cuda(semantics: str) -> int:
+    if semantics not in ("cpu", "cuda"):
+        raise ValueError(f"semantics must be 'cpu' or 'cuda', not {semantics!r}")
+    return 1 if semantics == "cuda" else 0
+
+
 def run(params: np.ndarray, pos, vel, radius, mass, nsteps: int, mesh=None, dtype="f64",
-        nthreads: int | None = None, fast: bool = False):
-    """Runs nsteps reference steps; returns (pos, vel) in `dtype`. fast: the mesh loop visits only
-    the triangles a triangle-grid index offers and the pair loop is split among threads (the
-    results of the reference's loops, much sooner); by default every triangle and a serial pair
-    loop, as the reference runs them."""
+        nthreads: int | None = None, fast: bool = False, semantics: str = "cpu"):
+    """Runs nsteps reference steps (semantics "cpu": the reference's CPU backend; "cuda": its CUDA
+    backend's step, the kicks and mesh terms added in ascending order); returns (pos, vel) in
+    `dtype`. fast: the mesh loop visits only the triangles a triangle-grid index offers and the
+    pair loop is split among threads (the results of the reference's loops, much sooner); by
+    default every triangle and a serial pair loop, as the reference runs them."""
     lib = load()
     t = np.float64 if dtype in ("f64", np.float64) else np.float32
     ct = C.c_double if t is np.float64 else C.c_float
@@ -101,7 +111,7 @@ def run(params: np.ndarray, pos, vel, radius, mass, nsteps: int, mesh=None, dtyp
     nthreads = nthreads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or lib.oracle_max_threads()
     rc = fn(_p(params, C.c_double), n, _p(pos, ct), _p(vel, ct), _p(r, ct), _p(m, ct), ntri,
             _p(tri, C.c_double) if tri is not None else None, int(nsteps), int(nthreads),
-            1 if fast else 0)
+            (1 if fast else 0) | (2 * _cuda(semantics)))
     if rc != 0:
         raise RuntimeError(f"oracle run failed ({rc})")
     return pos, vel
@@ -127,11 +137,12 @@ def bucketing(rmax: float, mn, mx, pos, dtype="f64"):
     return {"cell": cell, "count": count, "offset": offset, "perm": perm, "h": h, "n": n3}
 
 
-def closest_point(a, b, c, p):
+def closest_point(a, b, c, p, semantics: str = "cpu"):
     lib = load()
     arrs = [np.ascontiguousarray(x, np.float64) for x in (a, b, c, p)]
     q = np.zeros(3)
-    lib.oracle_closest_point_f64(*[_p(x, C.c_double) for x in arrs], _p(q, C.c_double))
+    fn = lib.oracle_closest_point_cuda_f64 if _cuda(semantics) else lib.oracle_closest_point_f64
+    fn(*[_p(x, C.c_double) for x in arrs], _p(q, C.c_double))
     return q
 
 

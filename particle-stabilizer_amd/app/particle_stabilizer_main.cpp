@@ -3,8 +3,10 @@
 // the backend is resolved the way the reference resolves its CUDA plugin: dlopen of the shared
 // library, dlsym of the runSimulate* symbol, call through the same function-pointer type.
 //
-// Differences, all deliberate: --hip selects this backend and is the default (the reference's CPU
-// engine is not part of this build; --cuda reports that the CUDA plugin is absent); --help works
+// Differences, all deliberate: --hip selects this backend and is the default, running the
+// reference CPU engine's step (its CPU engine is not part of this build); --cuda runs this backend
+// with the reference CUDA engine's step (PSX_FLAG_SEMANTICS_CUDA, passed to the plugin as
+// PSX_SEMANTICS=cuda, as --dtype is passed as PSX_DTYPE); --help works
 // like -h; --coeff-of-restitution-wall is read in its documented order Z_MIN Z_MAX X_MIN X_MAX
 // Y_MIN Y_MAX (the reference's parenthesised constructor call leaves the order unspecified and
 // g++ reverses it); a flag missing its value is an error instead of reading past argv;
@@ -69,8 +71,8 @@ void usage() {
          "[Options]\n"
          "  General\n"
          "    -h, --help                                                          Show this help message\n"
-         "    --hip                                                               Use the HIP backend (default)\n"
-         "    --cuda                                                              CUDA backend (not part of this build)\n"
+         "    --hip                                                               HIP backend, CPU engine's step (default)\n"
+         "    --cuda                                                              HIP backend, CUDA engine's step\n"
          "    --dtype                        f32|f64                              Device arithmetic (default f64)\n"
          "\n"
          "  Particle Constants\n"
@@ -207,7 +209,6 @@ int main(int argc, char** argv) {
             << ", restitutionSphere = " << args.coefficientOfRestitutionSphere
             << ", dt = " << args.dt << ", maxSteps = " << args.maxSteps << ", dtype = " << args.dtype
             << "\n";
-  if (args.toUseCUDA) die("the CUDA backend is not part of this build; use --hip");
   if (args.maxSteps < 0) die("--max-steps must be >= 0");
 
   // backend: dlopen + dlsym, as the reference loads its CUDA plugin (DynamicLoader.cpp:5-46)
@@ -246,6 +247,7 @@ int main(int argc, char** argv) {
 
   std::vector<Particle> result((size_t)n);
   setenv("PSX_DTYPE", args.dtype.c_str(), 1);
+  setenv("PSX_SEMANTICS", args.toUseCUDA ? "cuda" : "cpu", 1);
   std::cout << "[psx] Using HIP for the simulation.\n";
   run(particles, args.boxMin, args.boxMax, args.containerPath, args.gravity, args.wall,
       args.coefficientOfSpring, args.coefficientOfRestitutionSphere, args.dt,

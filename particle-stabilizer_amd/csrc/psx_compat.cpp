@@ -27,6 +27,13 @@ int dtype_from_env() {
   return PSX_F64;  // the reference's scalar type
 }
 
+// PSX_SEMANTICS=cuda: the reference CUDA engine's step (PSX_FLAG_SEMANTICS_CUDA); anything else,
+// or unset, its CPU engine's.
+int flags_from_env() {
+  const char* e = std::getenv("PSX_SEMANTICS");
+  return (e && std::strcmp(e, "cuda") == 0) ? PSX_FLAG_SEMANTICS_CUDA : 0;
+}
+
 }  // namespace
 
 extern "C" void runSimulateHIP(const std::vector<psxcompat::ParticlePtr>& particles,
@@ -71,6 +78,7 @@ extern "C" void runSimulateHIP(const std::vector<psxcompat::ParticlePtr>& partic
   prm.box_max[1] = boxMax.y;
   prm.box_max[2] = boxMax.z;
   prm.dtype = dtype_from_env();
+  prm.flags = flags_from_env();
 
   psx_mesh mesh;
   std::memset(&mesh, 0, sizeof(mesh));

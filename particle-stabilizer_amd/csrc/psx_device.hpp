@@ -73,6 +73,7 @@ struct SceneDev {
   int32_t p_begin, p_end;  // the scene's slice of the (sorted) particle arrays
   int32_t mesh;            // -1: box container; otherwise the scene has a triangle container
   int32_t tcell_base;      // first entry of the scene's cells in the triangle-cell offset table
+  int32_t cuda;            // 1: the reference CUDA backend's step (PSX_FLAG_SEMANTICS_CUDA)
 };
 
 // Triangle of a mesh container: corners and normal (PolygonContainer.cpp:52-68).
@@ -138,7 +139,16 @@ __device__ inline T dist_to(const Vec3<T>& p, const Vec3<T>& q) {
   return sqrt(dot3(dx, dy, dz, dx, dy, dz));
 }
 
-template <class T>
+// kDotPick: the edge point is picked by dot(p, q) instead of the distance |q - p|, as the
+// reference's CUDA backend does (getClosestPointOnPolygonKernelFunc,
+// PolygonContainerCUDA_kernels.cu:125-154); everything else is the same computation there.
+template <class T, bool kDotPick = false>
+__device__ inline T edge_metric(const Vec3<T>& p, const Vec3<T>& q) {
+  if constexpr (kDotPick) return dot3(p.x, p.y, p.z, q.x, q.y, q.z);
+  return dist_to(p, q);
+}
+
+template <class T, bool kDotPick = false>
 __device__ inline Vec3<T> closest_on_triangle(const TriGeo<T>& t, const Vec3<T>& p) {
   // projectPointOntoPlane(p, a, n)
   const T vx = p.x - t.a[0], vy = p.y - t.a[1], vz = p.z - t.a[2];
@@ -160,16 +170,16 @@ __device__ inline Vec3<T> closest_on_triangle(const TriGeo<T>& t, const Vec3<T>&
     if ((u >= T(0)) && (w >= T(0)) && (u + w <= T(1))) return proj;
   }
   const Vec3<T> qab = closest_on_segment(p, t.a, t.b);
-  T best = dist_to(p, qab);
+  T best = edge_metric<T, kDotPick>(p, qab);
   proj = qab;
   const Vec3<T> qbc = closest_on_segment(p, t.b, t.c);
-  const T dbc = dist_to(p, qbc);
+  const T dbc = edge_metric<T, kDotPick>(p, qbc);
   if (dbc < best) {
     best = dbc;
     proj = qbc;
   }
   const Vec3<T> qca = closest_on_segment(p, t.c, t.a);
-  const T dca = dist_to(p, qca);
+  const T dca = edge_metric<T, kDotPick>(p, qca);
   if (dca < best) {
     best = dca;
     proj = qca;

@@ -22,7 +22,7 @@ struct HostGrid {
   double h = 0;
   int64_t n[3] = {0, 0, 0};
 };
-HostGrid make_grid(double rmax, const double* mn, const double* mx);
+HostGrid make_grid(double rmax, const double* mn, const double* mx, bool cuda = false);
 struct MeshHost;
 void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32);
 
@@ -81,6 +81,7 @@ struct SimBase {
   int64_t tri_count = 0, tcell_entries = 0;
   std::vector<SceneHost> scenes;
   std::vector<MeshHost> meshes;
+  bool cuda = false;  // PSX_FLAG_SEMANTICS_CUDA: the reference CUDA backend's step
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   psx_tuning tuning = default_tuning();

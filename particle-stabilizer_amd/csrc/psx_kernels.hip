@@ -397,6 +397,110 @@ __global__ void __launch_bounds__(1024)
   });
 }
 
+// The mesh container of the reference's CUDA backend for one particle at its integrated position
+// x (PolygonContainerCUDA::resolveCollisions, PolygonContainerCUDA.cpp:130-214, with the kernels of
+// PolygonContainerCUDA_kernels.cu:159-294): a triangle is hit when its AABB overlaps the particle's
+// (<= and >=) and the point q that getClosestPointOnPolygonKernelFunc returns (the edge point
+// picked by dot(x, q)) has |q - x|^2 < r^2. The velocity becomes the mean of e (v - 2 (v.n) n)
+// over the hits, the position x + the sum of -(r - dot(-n, x - a)) n, both from the incoming x and
+// v. Only the list of x's grid cell is visited: it holds every triangle within reach of the cell,
+// which includes every triangle the test can hit. The reference adds the terms with atomics in no
+// fixed order; here they are added in ascending triangle order. modifyPositionKernel adds the sum
+// to every particle, hits or not, and so does this.
+template <class T>
+__device__ __forceinline__ void mesh_contact_cuda(const SceneDev<T>& s,
+                                                  const TriEntry<T>* __restrict__ tent,
+                                                  const int32_t* __restrict__ tcell_off,
+                                                  const uint8_t* __restrict__ cflags, Vec4<T>& x,
+                                                  Vec4<T>& v) {
+  const GridDev& g = s.grid;
+  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+  int32_t k0 = 0, k1 = 0;
+  if (cflags[c] & kCellTris) {
+    const int32_t tc = s.tcell_base + (c - g.cell_base);
+    k0 = tcell_off[tc];
+    k1 = tcell_off[tc + 1];
+  }
+  const Vec3<T> P = {x.x, x.y, x.z};
+  const T r = x.w;
+  const T lox = P.x - r, loy = P.y - r, loz = P.z - r;
+  const T hix = P.x + r, hiy = P.y + r, hiz = P.z + r;
+  T vbx = T(0), vby = T(0), vbz = T(0), mx = T(0), my = T(0), mz = T(0);
+  int32_t cnt = 0;
+  for (int32_t k = k0; k < k1; ++k) {  // isIntersectedKernel (PolygonContainerCUDA_kernels.cu:179-211)
+    const TriEntry<T>* __restrict__ te = tent + k;
+    if (!(lox <= te->hi[0] && hix >= te->lo[0] && loy <= te->hi[1] && hiy >= te->lo[1] &&
+          loz <= te->hi[2] && hiz >= te->lo[2]))
+      continue;
+    const TriGeo<T> tg = te->g;
+    const Vec3<T> q = closest_on_triangle<T, true>(tg, P);
+    const T rx = q.x - P.x, ry = q.y - P.y, rz = q.z - P.z;
+    if (!(dot3(rx, ry, rz, rx, ry, rz) < r * r)) continue;
+    // calcModifiedVelocitySumKernel (:214-236): e (v - (2 v.n) n)
+    const T tvn = T(2) * dot3(v.x, v.y, v.z, tg.n[0], tg.n[1], tg.n[2]);
+    vbx = vbx + s.ew * (v.x - tvn * tg.n[0]);
+    vby = vby + s.ew * (v.y - tvn * tg.n[1]);
+    vbz = vbz + s.ew * (v.z - tvn * tg.n[2]);
+    // calcPosModificationAmountKernel (:254-276): -(r - dot(-n, x - a)) n
+    const T sc = r - dot3(-tg.n[0], -tg.n[1], -tg.n[2], P.x - tg.a[0], P.y - tg.a[1], P.z - tg.a[2]);
+    mx = mx + (-sc) * tg.n[0];
+    my = my + (-sc) * tg.n[1];
+    mz = mz + (-sc) * tg.n[2];
+    ++cnt;
+  }
+  if (cnt > 0) {  // calcAveragedVelocityKernel (:238-252)
+    const T kc = (T)cnt;
+    v.x = vbx / kc;
+    v.y = vby / kc;
+    v.z = vbz / kc;
+  }
+  x.x = x.x + mx;  // modifyPositionKernel (:278-286)
+  x.y = x.y + my;
+  x.z = x.z + mz;
+}
+
+// K1 of the reference CUDA backend's step (PSX_FLAG_SEMANTICS_CUDA; PhysicsEngineCUDA.cpp:38-42):
+// integration (PhysicsEngine_kernels.cu:3-18, the same operations as the CPU backend), the container
+// (box walls as BoxContainer_kernels.cu:3-56, which match BoxContainer.cpp:22-65, or the CUDA mesh
+// contact), the state written back in place, then the cell and count of the post-container
+// position: membership and home cell both follow it (ParticleModelCUDA.cpp:122-177,
+// ParticleModel_kernels.cu:74-121). The reference's count kernel does not clamp the cell
+// (ParticleModel_kernels.cu:49-56), so a position outside the grid indexes outside its counters;
+// here it is clamped into the grid as the neighbour search clamps it (:61-76).
+template <class T, bool kTable, bool kBlocks>
+__global__ void __launch_bounds__(1024)
+    k_front(const Locator<T> loc, Vec4<T>* __restrict__ pos, Vec4<T>* __restrict__ vel,
+            const int32_t* __restrict__ id, const TriEntry<T>* __restrict__ tent,
+            const int32_t* __restrict__ tcell_off, const uint8_t* __restrict__ cflags,
+            int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
+            int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
+            uint32_t* __restrict__ ovf_ctr, const TileCounts tiles) {
+  __shared__ WgTiles wt;
+  with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
+    const int32_t p = p0 + (int32_t)threadIdx.x;
+    int32_t c = -1;
+    if (p < p1) {
+      Vec4<T> x = pos[p], v = vel[p];
+      const int32_t me = id[p];
+      integrate_position(s, x, v);
+      integrate_velocity(s, v);
+      if (s.mesh >= 0) mesh_contact_cuda(s, tent, tcell_off, cflags, x, v);
+      else box_walls(&s, x, v);
+      pos[p] = x;
+      vel[p] = v;
+      c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
+    }
+    if (tiles.tc) {
+      wt.begin(c);
+      wt.add(tiles, c);
+      wt.end(tiles);
+    }
+  });
+}
+
 // Dense cell lists (the scatter form of the bucketing): each particle takes a slot of its cell by
 // counting the cell's count back down (count - 1, ..., 0), and its id goes to offset[cell] + slot:
 // an unordered list per cell that the rank/gather kernel orders by id. The counts end at zero,
@@ -744,6 +848,7 @@ struct PairSelf {
   Vec4<T> x, v;
   T thr;  // (r_i + r_max)^2 (1 + margin): no candidate beyond it can be in contact
   T ks, es;
+  int32_t cuda;  // the scene's semantics flag (uniform): the CUDA backend's kick
 };
 
 // The prefilter of candidate j (fused multiply-adds, conservative by the margin): false only for
@@ -785,6 +890,19 @@ __device__ __forceinline__ void pair_force(const PairSelf<T>& me, const PairGeo<
   // d = x_i - x_j; dot(d, d) and length(d) equal dd and len bit for bit.
   const T dx = -g.ex, dy = -g.ey, dz = -g.ez;
   const T ov = g.R - g.len;
+  if (me.cuda) {
+    // resolveCollisionsKernel (ParticleModel_kernels.cu:163-199): the same contact test
+    // (overlap = r_i + r_j - sqrt(d.d) > 0 holds exactly when sqrt(d.d) < r_i + r_j), then
+    // (k ov / (|d| m_i) - 2 m_j (v_i - v_j).d e / ((m_i + m_j) d.d)) d
+    const T rvd = dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz);
+    const T crel = (((T(2) * vj.w) * rvd) * me.es) / ((me.v.w + vj.w) * g.dd);
+    const T cspr = (me.ks * ov) / (g.len * me.v.w);
+    const T f = cspr - crel;
+    fx = f * dx;
+    fy = f * dy;
+    fz = f * dz;
+    return;
+  }
   const T s3 = ((T(2) * vj.w) / (me.v.w + vj.w)) *
                (dot3(me.v.x - vj.x, me.v.y - vj.y, me.v.z - vj.z, dx, dy, dz) / g.dd);
   const T inv = T(1) / g.len;
@@ -833,6 +951,7 @@ __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
   me.thr = (rr * rr) * PairMargin<T>::value;
   me.ks = s->spring;
   me.es = s->es;
+  me.cuda = s->cuda;
   int32_t hx, hy, hz;
   cell_coords(g, home, hx, hy, hz);
   if (hw & kHomeCull) {  // all 27 neighbours exist and are bounded
@@ -1179,6 +1298,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
         me.thr = (rr * rr) * PairMargin<T>::value;
         me.ks = sc.spring;
         me.es = sc.es;
+        me.cuda = sc.cuda;
         int32_t hx, hy, hz;
         cell_coords(sc.grid, hc, hx, hy, hz);
         const uint32_t code = walk_code(sc, me.x, hw, hx, hy, hz);
@@ -1586,6 +1706,29 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
   return hipGetLastError();
 }
 
+template <class T>
+hipError_t launch_front(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+                        const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                        Vec4<T>* pos, Vec4<T>* vel, const int32_t* id, const TriEntry<T>* tent,
+                        const int32_t* tcell_off, const uint8_t* cflags, int32_t* cell_of,
+                        uint32_t* count, int32_t* cell_ids, uint32_t id_stride, int2* ovf,
+                        uint32_t* ovf_ctr, const TileCounts& tiles) {
+  const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
+  const dim3 grid(xcd_grid(ntab)), blk(block);
+#define PSX_FRONT(TABLE, BLOCKS)                                                                 \
+  PSX_LAUNCH((k_front<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, tent, tcell_off,   \
+             cflags, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr, tiles)
+  if (one) {
+    if (blocks) PSX_FRONT(false, true);
+    else PSX_FRONT(false, false);
+  } else {
+    if (blocks) PSX_FRONT(true, true);
+    else PSX_FRONT(true, false);
+  }
+#undef PSX_FRONT
+  return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1868,6 +2011,11 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
                                           const TileCounts&, int);                                     \
+  template hipError_t launch_front<T>(hipStream_t, int, int, const BlockInfo*,                  \
+                                      const SceneDev<T>*, const SceneDev<T>*, bool, Vec4<T>*,  \
+                                      Vec4<T>*, const int32_t*, const TriEntry<T>*,            \
+                                      const int32_t*, const uint8_t*, int32_t*, uint32_t*,     \
+                                      int32_t*, uint32_t, int2*, uint32_t*, const TileCounts&); \
   template hipError_t launch_rank_gather<T>(hipStream_t, int, int, const BlockInfo*,            \
                                             const SceneDev<T>*, const SceneDev<T>*, bool,      \
                                             int32_t, const int32_t*, const uint32_t*,          \

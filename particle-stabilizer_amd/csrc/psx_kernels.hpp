@@ -103,6 +103,18 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
                             const TileCounts& tiles, int mode);
 
+// The first kernel of a step with PSX_FLAG_SEMANTICS_CUDA: integration, the container (box walls or
+// the CUDA backend's mesh contact), the state written back to pos/vel, then the hash pass of the
+// post-container positions (cell, count, id slot; tile counts for the direct scan). cflags: the
+// per-cell flag byte (kCellTris).
+template <class T>
+hipError_t launch_front(hipStream_t st, int ntab, int block, const BlockInfo* btab,
+                        const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
+                        Vec4<T>* pos, Vec4<T>* vel, const int32_t* id, const TriEntry<T>* tent,
+                        const int32_t* tcell_off, const uint8_t* cflags, int32_t* cell_of,
+                        uint32_t* count, int32_t* cell_ids, uint32_t id_stride, int2* ovf,
+                        uint32_t* ovf_ctr, const TileCounts& tiles);
+
 // p[0, n) = 0 (a kernel, so that it may sit in a captured stream).
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n);
 

@@ -49,12 +49,13 @@ struct Error : std::runtime_error {
   } while (0)
 
 // ---------------------------------------------------------------------------------------------
-// grid geometry (ParticleBucket::initBucket, ParticleBucket.cpp:17-27)
+// grid geometry (ParticleBucket::initBucket, ParticleBucket.cpp:17-27; with cuda,
+// ParticleModelCUDA::initBuckets, ParticleModelCUDA.cpp:62-99, whose width uses an exact 1.05)
 // ---------------------------------------------------------------------------------------------
-HostGrid make_grid(double rmax, const double* mn, const double* mx) {
+HostGrid make_grid(double rmax, const double* mn, const double* mx, bool cuda) {
   HostGrid g;
   const double coeff = (double)1.05f;  // COEFF_FOR_INTERVAL_OF_BUCKETS, ParticleBucket.hpp:17
-  g.h = rmax * 2.0 * coeff;
+  g.h = cuda ? rmax * 2.0 * 1.05 : rmax * 2.0 * coeff;
   if (!(g.h > 0.0) || !std::isfinite(g.h))
     throw Error(PSX_ERR_INVALID, "largest radius must be positive and finite");
   for (int a = 0; a < 3; ++a) {
@@ -375,7 +376,7 @@ struct SimImpl final : SimBase {
           mx[a] = sh.prm.box_max[a];
         }
       }
-      sh.grid = make_grid(rmax, mn, mx);
+      sh.grid = make_grid(rmax, mn, mx, cuda);
       sh.cell_base = cells;
       const int64_t c = sh.grid.n[0] * sh.grid.n[1] * sh.grid.n[2];
       if (c > ((int64_t)1 << 30)) throw Error(PSX_ERR_INVALID, "grid of a scene exceeds 2^30 cells");
@@ -404,6 +405,7 @@ struct SimImpl final : SimBase {
       d.p_end = (int32_t)(sh.p_begin + sh.n);
       d.mesh = -1;
       d.tcell_base = 0;
+      d.cuda = cuda ? 1 : 0;
       if (sh.mesh >= 0) build_tri_grid(meshes[(size_t)sh.mesh], sh.grid, rmax, sizeof(T) == 4);
       {
         const std::vector<uint8_t> cs =
@@ -416,7 +418,8 @@ struct SimImpl final : SimBase {
         d.tcell_base = (int32_t)toff.size();
         const int32_t tri_base = (int32_t)tgeo.size();
         // triangle records (PolygonContainer.cpp:52-89): normal in double, stored in T; AABB over
-        // the T corners (glm::min / glm::max)
+        // the T corners (glm::min / glm::max). With cuda the normal is cross / length, component by
+        // component (PolygonContainerCUDA.cpp:64-82); the cross product is the same.
         std::vector<TriBox<T>> boxes((size_t)m.nt);
         for (int64_t t = 0; t < m.nt; ++t) {
           const double* q = &m.v[(size_t)(9 * t)];
@@ -425,8 +428,14 @@ struct SimImpl final : SimBase {
           // glm::cross(x, y) and glm::normalize(v) = v * (1 / sqrt(dot(v, v)))
           const D3 cr{e1.y * e2.z - e2.y * e1.z, e1.z * e2.x - e2.z * e1.x,
                       e1.x * e2.y - e2.x * e1.y};
-          const double inv = 1.0 / std::sqrt((cr.x * cr.x + cr.y * cr.y) + cr.z * cr.z);
-          const double nrm[3] = {cr.x * inv, cr.y * inv, cr.z * inv};
+          const double len = std::sqrt((cr.x * cr.x + cr.y * cr.y) + cr.z * cr.z);
+          const double inv = 1.0 / len;
+          double nrm[3] = {cr.x * inv, cr.y * inv, cr.z * inv};
+          if (cuda) {
+            nrm[0] = cr.x / len;
+            nrm[1] = cr.y / len;
+            nrm[2] = cr.z / len;
+          }
           TriGeo<T> tg;
           TriBox<T> tb;
           std::memset(&tb, 0, sizeof(tb));
@@ -675,7 +684,7 @@ struct SimImpl final : SimBase {
   bool hashed = false;
 
   void ensure_hashed(Profiler* prof) {
-    if (hashed) return;
+    if (hashed || cuda) return;  // the CUDA backend's step hashes after its container
     hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
          kModeIntegrate, prof);
     hashed = true;
@@ -692,8 +701,36 @@ struct SimImpl final : SimBase {
     hashed = false;
   }
 
+  // One step of the reference CUDA backend (PhysicsEngineCUDA.cpp:36-43): integration and the
+  // container in k_front, which also hashes the post-container state; scan; rank and gather (no
+  // integration or container there); the pair kick, with no hash pass of the next step (the next
+  // step's container comes first).
+  void step_once_cuda(Profiler* prof) {
+    const int a = cur_pos, vc = cur_vel;
+    int nb_i;
+    const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
+    const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
+    launch(0, prof, [&] {
+      return launch_front<T>(stream, nb_i, tuning.block_integrate, one ? nullptr : bt_i,
+                             scenes_d.as<SceneDev<T>>(), one, id_blocks(), pos[a].as<Vec4<T>>(),
+                             vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), tent_d.as<TriEntry<T>>(),
+                             tcell_off_d.as<int32_t>(), cull_d.as<uint8_t>(),
+                             cell_of.as<int32_t>(), count.as<uint32_t>(), cell_ids.as<int32_t>(),
+                             (uint32_t)id_stride, ovf.as<int2>(), ovf_ctr.as<uint32_t>(), tiles());
+    });
+    order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
+          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offs(), nullptr, kModeCheck, prof,
+          nullptr);
+    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offs(), vel[vc].as<Vec4<T>>(), prof);
+    cur_pos = 1 - a;
+  }
+
   // One step from a hashed state; leaves the next state hashed.
   void step_once(Profiler* prof) {
+    if (cuda) {
+      step_once_cuda(prof);
+      return;
+    }
     const int a = cur_pos, vc = cur_vel;
     ensure_hashed(prof);
     order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
@@ -744,6 +781,9 @@ struct SimImpl final : SimBase {
   }
 
   int64_t window_forward(int64_t nsteps, int64_t every) override {
+    if (cuda)
+      throw Error(PSX_ERR_STATE, "the differentiable window implements the CPU backend's step; this "
+                                 "handle was created with PSX_FLAG_SEMANTICS_CUDA");
     const size_t nn = (size_t)n;
     const int64_t nseg = (nsteps + every - 1) / every;
     if (nseg != win_nseg || every != win_every) {
@@ -950,6 +990,7 @@ struct SimImpl final : SimBase {
       if (p.box_min[a] != sh.prm.box_min[a] || p.box_max[a] != sh.prm.box_max[a])
         throw Error(PSX_ERR_INVALID, "psx_set_params cannot change the container bounds");
     if (p.dtype != sh.prm.dtype) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change dtype");
+    if (p.flags != sh.prm.flags) throw Error(PSX_ERR_INVALID, "psx_set_params cannot change flags");
     sh.prm = p;
     win_id = 0;   // a held window was run with the old coefficients
     drop_hash();  // hashed with the old dt
@@ -1299,7 +1340,8 @@ static int check_params(const psx_params* p) {
   if (!p) return set_error(PSX_ERR_INVALID, "params is NULL");
   if (p->dtype != PSX_F32 && p->dtype != PSX_F64) return set_error(PSX_ERR_INVALID, "dtype must be PSX_F32 or PSX_F64");
   if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return set_error(PSX_ERR_INVALID, "dt must be positive and finite");
-  if (p->flags != 0) return set_error(PSX_ERR_INVALID, "params.flags must be 0");
+  if (p->flags & ~PSX_FLAG_SEMANTICS_CUDA)
+    return set_error(PSX_ERR_INVALID, "params.flags: unknown bits (only PSX_FLAG_SEMANTICS_CUDA)");
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(p->gravity[a])) return set_error(PSX_ERR_INVALID, "gravity must be finite");
   return PSX_OK;
@@ -1318,6 +1360,8 @@ static int create_impl(int32_t n_scenes, const psx_scene* scenes, const double* 
     int rc = check_params(&scenes[s].params);
     if (rc) return rc;
     if (scenes[s].params.dtype != dtype) return set_error(PSX_ERR_INVALID, "all scenes must share one dtype");
+    if (scenes[s].params.flags != scenes[0].params.flags)
+      return set_error(PSX_ERR_INVALID, "all scenes must share one semantics (params.flags)");
     if (scenes[s].n <= 0) return set_error(PSX_ERR_INVALID, "every scene needs at least one particle");
     if (scenes[s].mesh.n_triangles < 0 || (scenes[s].mesh.n_triangles > 0 && !scenes[s].mesh.vertices))
       return set_error(PSX_ERR_INVALID, "mesh vertices missing");
@@ -1333,6 +1377,7 @@ static int create_impl(int32_t n_scenes, const psx_scene* scenes, const double* 
   PSX_TRY({
     auto impl = psx::make_sim(dtype);
     impl->n = total;
+    impl->cuda = (scenes[0].params.flags & PSX_FLAG_SEMANTICS_CUDA) != 0;
     int64_t p0 = 0;
     for (int32_t s = 0; s < n_scenes; ++s) {
       psx::SceneHost sh;

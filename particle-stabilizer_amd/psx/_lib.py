@@ -15,6 +15,7 @@ BIN_PATH = PKG_ROOT / "bin" / "particle_stabilizer_main"
 PSX_OK = 0
 PSX_F32 = 0
 PSX_F64 = 1
+PSX_FLAG_SEMANTICS_CUDA = 1  # psx_params.flags: the reference CUDA backend's step
 
 STATUS_NAMES = {0: "PSX_OK", 1: "PSX_ERR_INVALID", 2: "PSX_ERR_HIP", 3: "PSX_ERR_ALLOC",
                 4: "PSX_ERR_STATE", 5: "PSX_ERR_NUMERIC", 6: "PSX_ERR_IO"}

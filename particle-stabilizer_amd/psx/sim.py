@@ -14,7 +14,8 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import PSX_F32, PSX_F64, check, psx_mesh, psx_params, psx_scene, psx_tuning
+from ._lib import (PSX_F32, PSX_F64, PSX_FLAG_SEMANTICS_CUDA, check, psx_mesh, psx_params,
+                   psx_scene, psx_tuning)
 
 _DTYPES = {"f32": PSX_F32, "float32": PSX_F32, "f64": PSX_F64, "float64": PSX_F64}
 
@@ -53,6 +54,9 @@ class Params:
     box_min: Sequence[float] = (0.0, 0.0, 0.0)
     box_max: Sequence[float] = (10.0, 20.0, 30.0)
     dtype: str = "f64"
+    # "cpu": the reference CPU backend's step (PhysicsEngine.cpp:38-48); "cuda": its CUDA backend's
+    # (PhysicsEngineCUDA.cpp:36-43), PSX_FLAG_SEMANTICS_CUDA
+    semantics: str = "cpu"
 
     def to_c(self) -> psx_params:
         p = psx_params()
@@ -64,7 +68,9 @@ class Params:
         p.box_min[:] = [float(x) for x in self.box_min]
         p.box_max[:] = [float(x) for x in self.box_max]
         p.dtype = _dtype_code(self.dtype)
-        p.flags = 0
+        if self.semantics not in ("cpu", "cuda"):
+            raise ValueError(f"semantics must be 'cpu' or 'cuda', not {self.semantics!r}")
+        p.flags = PSX_FLAG_SEMANTICS_CUDA if self.semantics == "cuda" else 0
         return p
 
     def as_array(self) -> np.ndarray:

@@ -45,7 +45,7 @@ def small_cylinder(n=800, seed=1, dtype="f64", radius=6.0, height=20.0, n_az=24,
 def oracle_run(oracle, sc: Scene, steps: int, dtype=None):
     dtype = dtype or sc.params.dtype
     return oracle.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps,
-                      mesh=sc.mesh, dtype=dtype)
+                      mesh=sc.mesh, dtype=dtype, semantics=sc.params.semantics)
 
 
 def hip_run(sc: Scene, steps: int, graph=False, **tuning):

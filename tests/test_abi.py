@@ -102,3 +102,30 @@ def test_null_handles_rejected():
     assert lib.psx_kernel_name(4) == b"k_scatter_ids"
     assert lib.psx_kernel_name(5) is None
     assert lib.psx_kernel_name(99) is None
+
+
+def test_semantics_flags_checked():
+    """psx_params.flags takes PSX_FLAG_SEMANTICS_CUDA and nothing else, and a batch's scenes must
+    agree on it (both refused before any device work)."""
+    assert _lib.PSX_FLAG_SEMANTICS_CUDA == 1
+    assert Params(semantics="cuda").to_c().flags == 1 and Params().to_c().flags == 0
+    with pytest.raises(ValueError):
+        Params(semantics="gpu").to_c()
+    lib = _lib.load()
+    n = 4
+    pos = np.zeros((2 * n, 3)) + 1.0
+    vel = np.zeros((2 * n, 3))
+    r = np.full(2 * n, 0.5)
+    m = np.full(2 * n, 1.0)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    bad = Params().to_c()
+    bad.flags = 2
+    h = C.c_void_p()
+    rc = lib.psx_create(C.byref(bad), n, dp(pos), dp(vel), dp(r), dp(m), None, C.byref(h))
+    assert rc == 1 and "flags" in lib.psx_last_error().decode() and not h.value
+    scenes = (_lib.psx_scene * 2)()
+    scenes[0].params = Params(semantics="cuda").to_c()
+    scenes[1].params = Params().to_c()
+    scenes[0].n = scenes[1].n = n
+    rc = lib.psx_batch_create(2, scenes, dp(pos), dp(vel), dp(r), dp(m), C.byref(h))
+    assert rc == 1 and "semantics" in lib.psx_last_error().decode() and not h.value

@@ -14,10 +14,13 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
+@pytest.mark.parametrize("backend,semantics", [("--hip", "cpu"), ("--cuda", "cuda")])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_cli_sample_matches_oracle(oracle, tmp_path, dtype):
+def test_cli_sample_matches_oracle(oracle, tmp_path, dtype, backend, semantics):
+    """--hip runs the reference CPU engine's step, --cuda its CUDA engine's step
+    (PSX_FLAG_SEMANTICS_CUDA)."""
     out = tmp_path / "out" / "result.csv"
-    args = [str(BIN_PATH), str(GOLDEN / "sample.csv"), "--hip", "--dtype", dtype,
+    args = [str(BIN_PATH), str(GOLDEN / "sample.csv"), backend, "--dtype", dtype,
             "--initial-velocity", "0", "0", "-10", "--gravity", "0", "0", "-10",
             "--box-min", "-20", "-20", "-20", "--box-max", "20", "20", "20",
             "--coeff-of-spring", "50", "--coeff-of-restitution-sphere", "0.3",
@@ -30,7 +33,7 @@ def test_cli_sample_matches_oracle(oracle, tmp_path, dtype):
     prm = Params(gravity=(0, 0, -10), walls=(0.5,) * 6, spring=50.0, restitution=0.3, dt=1e-3,
                  box_min=(-20, -20, -20), box_max=(20, 20, 20))
     rp, _ = oracle.run(prm.as_array(), d["position"], d["velocity"], d["radius"], d["mass"], 200,
-                       dtype=dtype)
+                       dtype=dtype, semantics=semantics)
     expect = tmp_path / "expect.csv"
     io.write_csv(expect, rp.astype(np.float64), d["radius"])
     assert out.read_bytes() == expect.read_bytes()
