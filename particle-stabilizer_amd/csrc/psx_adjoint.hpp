@@ -6,7 +6,9 @@
 namespace psx {
 
 // Lanes per particle of the adjoint step kernel (its workgroups hold 256 / kAdjLanes particles).
-constexpr int kAdjLanes = 8;
+// Measured on MI355X over the 16,384-sphere calibration window, per adjoint step: 1 lane 36.6 us,
+// 2 lanes 27.6, 4 lanes 22.5, 8 lanes 23.6, 16 lanes 35.3.
+constexpr int kAdjLanes = 4;
 constexpr int kAdjPerBlock = 256 / kAdjLanes;
 
 // Adjoint of one step (pair, container and integrate terms) for the bucket-sorted snapshot of the
