@@ -109,3 +109,24 @@ def test_cuda_window_refused(psx_mod):
         s.step(3)  # the handle still steps
         p, _ = s.state()
         assert np.isfinite(p).all()
+
+
+@pytest.mark.parametrize("workload", ["cylinder_settle", "hopper_settle"])
+def test_cuda_full_size_workloads_bitexact(psx_mod, oracle, workload):
+    """The 262,144-sphere cylinder (2,176 triangles) and the 4,194,304-sphere hopper (8,192
+    triangles) in the CUDA engine's step, binary32, with their tuned launch shapes: 3 steps equal
+    the oracle's (fast mode: the results of its reference loops)."""
+    import json
+    sc = _cuda(getattr(scenes, workload)())
+    n = sc.radius.shape[0]
+    cache = json.loads(open(scenes.__file__.replace("scenes.py", "tuning_mi355x.json")).read())
+    tuning = next(e["tuning"] for k, e in cache.items() if f"|n={n}|" in k)
+    steps = 3
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
+        s.set_tuning(**dict(tuning, use_graph=0))
+        s.step(steps)
+        p, v = s.state()
+    rp, rv = oracle.run(sc.params.as_array(), sc.pos, sc.vel, sc.radius, sc.mass, steps,
+                        mesh=sc.mesh, dtype="f32", fast=True, semantics="cuda")
+    assert_bitwise(as_dtype(p, "f32"), rp, f"{workload} pos")
+    assert_bitwise(as_dtype(v, "f32"), rv, f"{workload} vel")
