@@ -1011,6 +1011,56 @@ __device__ __forceinline__ void pair_contact(const PairSelf<T>& me, const Vec4<T
   }
 }
 
+// Near candidates deferred to a per-lane queue (the one-lane kernel). A chunk whose prefilter
+// passes in any lane runs the exact test (a correctly rounded square root and three divisions) and
+// the kick for the whole wave; in contact-rich states most chunks of most waves do. The one-lane
+// kernel instead files each near candidate's index in its lane's queue in LDS, in walk order, and
+// runs the queue's exact tests in one loop after the walk: the wave runs as many exact tests as its
+// lane with the most near candidates. The kicks depend on the snapshot only (Jacobi), and the
+// additions keep the walk's order, so the results are bit-identical. The walk visits candidates in
+// ascending sorted index (columns in ascending cell order, each slice ascending), so a lane whose
+// queue fills (crowded cells) keeps the first index it could not file, and after the queue it tests
+// the candidates from that index on in line: nothing is walked twice, and the order holds.
+// (Emptying the queue inside the walk would put a copy of the exact tests at every filing site of
+// the unrolled walk, and walking again would put the in-line walk's registers into the kernel.)
+constexpr int kNearQ = 16;  // entries per lane
+
+struct NearQueue {
+  uint32_t* slot;  // this lane's entry 0 in LDS; entry k at slot[k * stride]
+  uint32_t stride; // lanes of the workgroup
+  uint32_t n;      // entries held
+  uint32_t rest;   // the first near candidate not filed (~0u: none)
+};
+
+__device__ __forceinline__ void near_push(NearQueue& nq, uint32_t j) {
+  if (nq.n < (uint32_t)kNearQ) {
+    nq.slot[nq.n * nq.stride] = j;
+    ++nq.n;
+  } else if (nq.rest == ~0u) {
+    nq.rest = j;
+  }
+}
+
+// The exact tests and kicks of the queued candidates in queue order. Each entry's position and
+// velocity are loaded together.
+template <class T>
+__device__ __forceinline__ void near_flush(const NearQueue& nq, const PairSelf<T>& me,
+                                           const Vec4<T>* __restrict__ pos,
+                                           const Vec4<T>* __restrict__ vsn, T& ax, T& ay, T& az) {
+  for (uint32_t k = 0; k < nq.n; ++k) {
+    const uint32_t j = nq.slot[k * nq.stride];
+    const Vec4<T> xj = pos[j], vj = vsn[j];
+    PairGeo<T> g;
+    if (pair_touch(me, xj, xj.w, g)) {
+      T fx, fy, fz;
+      pair_force(me, g, vj, fx, fy, fz);
+      ax = ax + fx;
+      ay = ay + fy;
+      az = az + fz;
+    }
+  }
+}
+
 // Chunks of four candidates: one address, four independent loads, four prefilters and a single
 // branch into the rare path, which applies the contacts of the chunk in ascending order. The
 // masks are combined with bitwise operators, so every prefilter runs and the compiler issues the
@@ -1098,8 +1148,62 @@ __device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>&
   }
 }
 
-// The contact sum of particle p over its nine column slices, in slice order, W (4 or 2) candidates
-// per round. In binary32 the first chunk of the next slice is loaded while the current slice is
+// Slice [lo, hi) from index `from` on, the exact tests in line (a lane whose queue filled).
+template <class T>
+__device__ __forceinline__ void near_tail(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                          const Vec4<T>* __restrict__ vsn, uint32_t lo, uint32_t hi,
+                                          uint32_t from, uint32_t self, T& ax, T& ay, T& az) {
+  for (uint32_t j = max(lo, from); j < hi; ++j) {
+    const Vec4<T> xj = pos[j];
+    if (j != self && pair_near(me, xj)) pair_contact(me, xj, vsn, j, ax, ay, az);
+  }
+}
+
+// The chunk tests of the one-lane kernel: the near candidates go to the lane's queue.
+template <class T>
+__device__ __forceinline__ void pair_test4q(const PairSelf<T>& me, const Vec4<T> (&c)[4],
+                                            uint32_t q, uint32_t e, uint32_t self, NearQueue& nq) {
+  bool m[4];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t j = q + (uint32_t)k;
+    m[k] = (j < e) & (j != self) & pair_near(me, c[k]);
+    any = any | m[k];
+  }
+  if (any) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (m[k]) near_push(nq, q + (uint32_t)k);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void pair_chunk4q(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
+                                             uint32_t q, uint32_t e, uint32_t self, NearQueue& nq) {
+  Vec4<T> c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c[k] = Vec4<T>{T(0), T(0), T(0), T(0)};
+    if (q + (uint32_t)k < e) c[k] = pos[q + k];
+  }
+  pair_test4q(me, c, q, e, self, nq);
+}
+
+template <class T>
+__device__ __forceinline__ void pair_test2q(const PairSelf<T>& me, const Vec4<T>& c0,
+                                            const Vec4<T>& c1, uint32_t q, uint32_t e,
+                                            uint32_t self, NearQueue& nq) {
+  const bool m0 = q < e && q != self && pair_near(me, c0);
+  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
+  if (m0 || m1) {
+    if (m0) near_push(nq, q);
+    if (m1) near_push(nq, q + 1);
+  }
+}
+
+// The walk of particle p over its nine column slices, in slice order, W (4 or 2) candidates per
+// round, its near candidates filed in the lane's queue. In binary32 the first chunk of the next slice is loaded while the current slice is
 // tested, so the slices' loads overlap instead of forming a chain of nine round trips (binary64
 // lacks the registers). Measured on MI355X: prefetching two or three slices ahead changes the
 // 262k-, 4.2M- and 4.6M-sphere workloads by about 1 % either way (more registers, fewer waves per
@@ -1107,9 +1211,8 @@ __device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>&
 // slower on the 262k-sphere cylinder, so the chunk width is a tuning choice (psx_tuning.pair_chunk).
 template <class T, int W>
 __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
-                                         const Vec4<T>* __restrict__ vsn,
                                          const uint32_t (&lo)[9], const uint32_t (&hi)[9],
-                                         uint32_t self, T& ax, T& ay, T& az) {
+                                         uint32_t self, NearQueue& nq) {
   static_assert(W == 2 || W == 4, "two or four candidates per round");
   if constexpr (sizeof(T) == 4) {
     Vec4<T> buf[W];
@@ -1128,13 +1231,13 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
       const uint32_t e = hi[d];
       // unconditional (the masks cover an empty slice), so the prefetched chunk is used as loaded
       if constexpr (W == 4) {
-        pair_test4(me, c, vsn, lo[d], e, self, ax, ay, az);
-        for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk4(me, pos, vsn, q, e, self, ax, ay, az);
+        pair_test4q(me, c, lo[d], e, self, nq);
+        for (uint32_t q = lo[d] + 4; q < e; q += 4) pair_chunk4q(me, pos, q, e, self, nq);
       } else {
-        pair_test2(me, c[0], c[1], vsn, lo[d], e, self, ax, ay, az);
+        pair_test2q(me, c[0], c[1], lo[d], e, self, nq);
         for (uint32_t q = lo[d] + 2; q < e; q += 2) {
           const Vec4<T>* __restrict__ src = pos + q;
-          pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+          pair_test2q(me, src[0], src[1], q, e, self, nq);
         }
       }
     }
@@ -1143,11 +1246,11 @@ __device__ __forceinline__ void pair_sum(const PairSelf<T>& me, const Vec4<T>* _
     for (int d = 0; d < 9; ++d) {
       const uint32_t e = hi[d];
       if constexpr (W == 4) {
-        for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk4(me, pos, vsn, q, e, self, ax, ay, az);
+        for (uint32_t q = lo[d]; q < e; q += 4) pair_chunk4q(me, pos, q, e, self, nq);
       } else {
         for (uint32_t q = lo[d]; q < e; q += 2) {
           const Vec4<T>* __restrict__ src = pos + q;
-          pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+          pair_test2q(me, src[0], src[1], q, e, self, nq);
         }
       }
     }
@@ -1206,14 +1309,15 @@ __device__ __forceinline__ uint32_t walk_code(const SceneDev<T>& s, const Vec4<T
   return code;
 }
 
-// Offsets of cells z - 1 .. z + 2 of column d around home cell `home` (zeros for a column not walked).
+// Offsets of cells z - 1 .. z + 2 of column d around home cell `home`; the lanes of a column not
+// walked all read the first entries of the array (one cache line; discarded by walk_bounds). The
+// load is unconditional: a load under a branch is merged with its default afterwards, and the
+// compiler waited for all loads in flight (the prefetched chunk among them) at that merge.
 template <int d>
 __device__ __forceinline__ Off4 walk_offsets(const uint32_t* __restrict__ offset, int32_t home,
                                              int32_t ny, int32_t nz, uint32_t code) {
-  Off4 o{0u, 0u, 0u, 0u};
-  if (code & (1u << d))
-    o = *reinterpret_cast<const Off4*>(offset + home + ((d / 3 - 1) * ny + (d % 3 - 1)) * nz - 1);
-  return o;
+  const int32_t c = (code & (1u << d)) ? home + ((d / 3 - 1) * ny + (d % 3 - 1)) * nz : 1;
+  return *reinterpret_cast<const Off4*>(offset + c - 1);
 }
 
 template <int d>
@@ -1224,15 +1328,27 @@ __device__ __forceinline__ void walk_bounds(const Off4& o, uint32_t code, uint32
   if (!(code & (1u << d))) lo = hi = 0u;
 }
 
+// Column d's slice [lo, hi) for the walk code (d a constant after unrolling).
+__device__ __forceinline__ void code_bounds(const uint32_t* __restrict__ offset, int32_t home,
+                                            int32_t ny, int32_t nz, uint32_t code, int d,
+                                            uint32_t& lo, uint32_t& hi) {
+  lo = hi = 0u;
+  if (code & (1u << d)) {
+    const Off4 o =
+        *reinterpret_cast<const Off4*>(offset + home + ((d / 3 - 1) * ny + (d % 3 - 1)) * nz - 1);
+    lo = (code & (1u << (9 + d))) ? o.a : o.b;
+    hi = (code & (1u << (18 + d))) ? o.d : o.c;
+  }
+}
+
 // Slice d: its first chunk (buf, loaded one slice earlier), then the rest; meanwhile slice d + 1's
 // bounds are decoded and its first chunk loaded, and slice d + 2's offsets fetched.
 template <int d, class T>
 __device__ __forceinline__ void walk_slice(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
-                                           const Vec4<T>* __restrict__ vsn,
                                            const uint32_t* __restrict__ offset, int32_t home,
                                            int32_t ny, int32_t nz, uint32_t code, uint32_t self,
                                            uint32_t& lo, uint32_t& hi, Vec4<T> (&buf)[2], Off4& o2,
-                                           T& ax, T& ay, T& az) {
+                                           NearQueue& nq) {
   const Vec4<T> c0 = buf[0], c1 = buf[1];
   const uint32_t l = lo, e = hi;
   if constexpr (d + 1 < 9) {
@@ -1241,45 +1357,45 @@ __device__ __forceinline__ void walk_slice(const PairSelf<T>& me, const Vec4<T>*
     buf[1] = pos[lo + 1];
   }
   if constexpr (d + 2 < 9) o2 = walk_offsets<d + 2>(offset, home, ny, nz, code);
-  pair_test2(me, c0, c1, vsn, l, e, self, ax, ay, az);
+  pair_test2q(me, c0, c1, l, e, self, nq);
   for (uint32_t q = l + 2; q < e; q += 2) {
     const Vec4<T>* __restrict__ src = pos + q;
-    pair_test2(me, src[0], src[1], vsn, q, e, self, ax, ay, az);
+    pair_test2q(me, src[0], src[1], q, e, self, nq);
   }
 }
 
 template <class T>
 __device__ __forceinline__ void pair_sum_coded(const SceneDev<T>& sc, const PairSelf<T>& me,
                                                const Vec4<T>* __restrict__ pos,
-                                               const Vec4<T>* __restrict__ vsn,
                                                const uint32_t* __restrict__ offset, int32_t home,
-                                               uint32_t code, uint32_t self, T& ax, T& ay, T& az) {
+                                               uint32_t code, uint32_t self, NearQueue& nq) {
   const int32_t ny = sc.grid.n[1], nz = sc.grid.n[2];
   uint32_t lo, hi;
   Off4 o2 = walk_offsets<0>(offset, home, ny, nz, code);
   walk_bounds<0>(o2, code, lo, hi);
   o2 = walk_offsets<1>(offset, home, ny, nz, code);
   Vec4<T> buf[2] = {pos[lo], pos[lo + 1]};
-  walk_slice<0>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<1>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<2>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<3>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<4>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<5>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<6>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<7>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
-  walk_slice<8>(me, pos, vsn, offset, home, ny, nz, code, self, lo, hi, buf, o2, ax, ay, az);
+  walk_slice<0>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<1>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<2>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<3>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<4>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<5>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<6>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<7>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
+  walk_slice<8>(me, pos, offset, home, ny, nz, code, self, lo, hi, buf, o2, nq);
 }
 
-// The binary32 two-wide form is held to eight waves per SIMD (at most 64 VGPRs; it needs 67
-// unbounded, and spills nothing bounded): it runs on the workloads that fill the GPU many times
-// over, where more resident waves hide more of each wave's chain of slice loads (1 % on the
-// 4.2M-sphere hopper and the 256-scene batch, measured on MI355X).
+// The binary32 two-wide form is held to eight waves per SIMD (at most 64 VGPRs; it takes 53 with
+// the near-candidate queue): it runs on the workloads that fill the GPU many times over, where
+// more resident waves hide more of each wave's chain of slice loads (1 % on the 4.2M-sphere hopper
+// and the 256-scene batch, measured on MI355X, when it needed 67 unbounded).
 template <class T, bool kTable, int W>
 __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
     k_pairs(const Locator<T> loc, const Vec4<T>* __restrict__ pos,
             const Vec4<T>* __restrict__ vsn, const uint32_t* __restrict__ offset,
             const uint32_t* __restrict__ home, Vec4<T>* __restrict__ vout, const NextBucket next) {
+  extern __shared__ uint32_t near_raw[];  // kNearQ entries per lane
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -1288,6 +1404,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
     int32_t c = -1;
     if (p < p1) {
       PairSelf<T> me;
+      NearQueue nq{near_raw + threadIdx.x, blockDim.x, 0u, ~0u};
       T ax, ay, az;
       if constexpr (W == 2 && sizeof(T) == 4) {
         me.x = pos[p];
@@ -1305,14 +1422,29 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
         ax = me.v.x;
         ay = me.v.y;
         az = me.v.z;
-        pair_sum_coded(sc, me, pos, vsn, offset, hc, code, (uint32_t)p, ax, ay, az);
+        pair_sum_coded(sc, me, pos, offset, hc, code, (uint32_t)p, nq);
+        near_flush(nq, me, pos, vsn, ax, ay, az);
+        if (nq.rest != ~0u) {  // the queue filled: the candidates from its first one left out on
+#pragma unroll
+          for (int d = 0; d < 9; ++d) {
+            uint32_t lo, hi;
+            code_bounds(offset, hc, sc.grid.n[1], sc.grid.n[2], code, d, lo, hi);
+            near_tail(me, pos, vsn, lo, hi, nq.rest, (uint32_t)p, ax, ay, az);
+          }
+        }
       } else {
         uint32_t lo[9], hi[9];
         pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
         ax = me.v.x;
         ay = me.v.y;
         az = me.v.z;
-        pair_sum<T, W>(me, pos, vsn, lo, hi, (uint32_t)p, ax, ay, az);
+        pair_sum<T, W>(me, pos, lo, hi, (uint32_t)p, nq);
+        near_flush(nq, me, pos, vsn, ax, ay, az);
+        if (nq.rest != ~0u) {  // the queue filled: the candidates from its first one left out on
+#pragma unroll
+          for (int d = 0; d < 9; ++d)
+            near_tail(me, pos, vsn, lo[d], hi[d], nq.rest, (uint32_t)p, ax, ay, az);
+        }
       }
       const Vec4<T> vn{ax, ay, az, me.v.w};
       vout[p] = vn;
@@ -1920,12 +2052,14 @@ static hipError_t launch_pairs_t(hipStream_t st, const Locator<T>& loc, int bloc
                                  const NextBucket& next) {
   const dim3 grid(xcd_grid(loc.ntab)), blk(block);
   switch (lanes) {
-    case 1:
+    case 1: {
+      const size_t shm = (size_t)kNearQ * (size_t)block * sizeof(uint32_t);
       if (chunk == 2)
-        PSX_LAUNCH((k_pairs<T, kTable, 2>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
+        PSX_LAUNCH((k_pairs<T, kTable, 2>), grid, blk, shm, st, loc, pos, vsn, offset, home, vout, next);
       else
-        PSX_LAUNCH((k_pairs<T, kTable, 4>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
+        PSX_LAUNCH((k_pairs<T, kTable, 4>), grid, blk, shm, st, loc, pos, vsn, offset, home, vout, next);
       break;
+    }
     case 2:
       PSX_LAUNCH((k_pairs_lanes<T, 2, kTable>), grid, blk, 0, st, loc, pos, vsn, offset, home, vout, next);
       break;

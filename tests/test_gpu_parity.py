@@ -91,6 +91,12 @@ def test_bucketing_cell_overflow(psx_mod, oracle, dtype):
         p, v = hip_run(sc, 3, sort_variant=variant)
         assert_bitwise(as_dtype(p, dtype), ref_p, f"pos sort_variant={variant}")
         assert_bitwise(as_dtype(v, dtype), ref_v, f"vel sort_variant={variant}")
+    # the one-lane kernel's near-candidate queue overflows in the 300-sphere cell: the lanes test
+    # the candidates after the queue's last one in line (four- and two-wide chunks)
+    for chunk in (4, 2):
+        p, v = hip_run(sc, 3, pair_variant=1, pair_chunk=chunk)
+        assert_bitwise(as_dtype(p, dtype), ref_p, f"pos pair_chunk={chunk}")
+        assert_bitwise(as_dtype(v, dtype), ref_v, f"vel pair_chunk={chunk}")
     # the LDS-staged pair kernel: the 300-sphere cell overfills a workgroup's LDS share, so some
     # workgroups read from memory and the others from LDS
     p, v = hip_run(sc, 3, pair_variant=32, block_pairs=64)
