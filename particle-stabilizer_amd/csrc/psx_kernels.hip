@@ -1041,23 +1041,38 @@ __device__ __forceinline__ void near_push(NearQueue& nq, uint32_t j) {
   }
 }
 
-// The exact tests and kicks of the queued candidates in queue order. Each entry's position and
-// velocity are loaded together.
+// The exact test and kick of candidate j (position xj, velocity vj) on particle i, added to the sum.
+template <class T>
+__device__ __forceinline__ void near_add(const PairSelf<T>& me, const Vec4<T>& xj,
+                                         const Vec4<T>& vj, T& ax, T& ay, T& az) {
+  PairGeo<T> g;
+  if (pair_touch(me, xj, xj.w, g)) {
+    T fx, fy, fz;
+    pair_force(me, g, vj, fx, fy, fz);
+    ax = ax + fx;
+    ay = ay + fy;
+    az = az + fz;
+  }
+}
+
+// The exact tests and kicks of the queued candidates in queue order, two entries per round with
+// two register sets: the next entry's position and velocity are loaded before an entry is tested.
 template <class T>
 __device__ __forceinline__ void near_flush(const NearQueue& nq, const PairSelf<T>& me,
                                            const Vec4<T>* __restrict__ pos,
                                            const Vec4<T>* __restrict__ vsn, T& ax, T& ay, T& az) {
-  for (uint32_t k = 0; k < nq.n; ++k) {
-    const uint32_t j = nq.slot[k * nq.stride];
-    const Vec4<T> xj = pos[j], vj = vsn[j];
-    PairGeo<T> g;
-    if (pair_touch(me, xj, xj.w, g)) {
-      T fx, fy, fz;
-      pair_force(me, g, vj, fx, fy, fz);
-      ax = ax + fx;
-      ay = ay + fy;
-      az = az + fz;
-    }
+  if (nq.n == 0u) return;
+  const uint32_t last = nq.n - 1u;
+  uint32_t j = nq.slot[0];
+  Vec4<T> xa = pos[j], va = vsn[j];
+  for (uint32_t k = 0; k < nq.n; k += 2) {
+    j = nq.slot[min(k + 1u, last) * nq.stride];
+    const Vec4<T> xb = pos[j], vb = vsn[j];
+    near_add(me, xa, va, ax, ay, az);
+    j = nq.slot[min(k + 2u, last) * nq.stride];
+    xa = pos[j];
+    va = vsn[j];
+    if (k + 1u < nq.n) near_add(me, xb, vb, ax, ay, az);
   }
 }
 
