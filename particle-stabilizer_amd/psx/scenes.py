@@ -78,6 +78,18 @@ def cylinder_settle(n: int = 262144, seed: int = 0, steps: int = 5000, dtype: st
                        "triangles": int(mesh.shape[0]), "generator": GENERATOR})
 
 
+def cylinder_packed(n: int = 262144, seed: int = 0, steps: int = 1000, dtype: str = "f32",
+                    factor: float = 0.6) -> Scene:
+    """The cylinder settle with its lattice heights compressed by `factor` about z = 1 (0.6:
+    vertical spacing 1.02, so every sphere's vertical neighbours are within the pair prefilter's
+    reach and 0.88 pairs per sphere touch): a contact-rich state like a settled pile's, for timing
+    the pair kernel's exact tests."""
+    sc = cylinder_settle(n=n, seed=seed, steps=steps, dtype=dtype)
+    sc.pos[:, 2] = 1.0 + (sc.pos[:, 2] - 1.0) * factor
+    sc.meta.update(name="cylinder_packed", height_factor=factor)
+    return sc
+
+
 def hopper_settle(n: int = 4194304, seed: int = 0, steps: int = 2000, dtype: str = "f32") -> Scene:
     """Cone-bottom hopper (outlet R 10, cone half-angle 35 deg to R 150, cylinder height 400,
     8,192 triangles); r in {0.4 (70%), 0.8 (30%)}; lattice spacing 1.8, v0 (0,0,-1); spring 2e4;

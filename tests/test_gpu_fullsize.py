@@ -60,8 +60,7 @@ def test_compressed_cylinder_steps_bitexact(psx_mod, oracle, chunk):
     0.88 contacts per sphere, and the vertical neighbours of every sphere within the prefilter's
     reach): the pair kernel's rare path runs on most spheres. Binary32, bit-exact against the
     oracle, with four- and two-wide chunks."""
-    sc = scenes.cylinder_settle()
-    sc.pos[:, 2] = 1.0 + (sc.pos[:, 2] - 1.0) * 0.6
+    sc = scenes.cylinder_packed()
     steps = 3
     with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
         s.set_tuning(use_graph=1, pair_variant=1, pair_chunk=chunk)
