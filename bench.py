@@ -206,11 +206,12 @@ def bench_plain(workload: str, steps: int, warmup: int, profile_steps: int, dtyp
             "kernels": kernels, "roofline": roofline, "stats": stats}
 
 
-def bench_batched(steps: int = 128, warmup: int = 10) -> dict:
+def bench_batched(steps: int = 128, warmup: int = 10, graph: bool = True) -> dict:
     """256 independent box scenes of uneven size: one launch per kernel per step (hipGraph)
     against the same kernels run once per scene -- launched step by step (on one stream, and on
     one stream per scene) and replayed from each scene's own 64-step graphs (one stream per
-    scene). Every graph is captured before its clock starts."""
+    scene). Every graph is captured before its clock starts. graph=False launches every kernel
+    on its own (the batch included) and skips the per-scene graphs."""
     import ctypes
 
     import psx
@@ -219,14 +220,14 @@ def bench_batched(steps: int = 128, warmup: int = 10) -> dict:
     total = sum(sc.radius.shape[0] for sc in scs)
     b = psx.Batch(scs)
     tuning = tuned(b)
-    b.set_tuning(use_graph=1)
+    b.set_tuning(use_graph=1 if graph else 0)
     b.step(warmup)
     tb = _timed(b, steps)
     cells = b.num_cells
     b.close()
     sims = [psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) for sc in scs]
     for s in sims:
-        s.set_tuning(**tuning)
+        s.set_tuning(**{**tuning, "use_graph": 1 if graph else 0})
         s.step(warmup)
     for s in sims:
         s.synchronize()
@@ -252,21 +253,23 @@ def bench_batched(steps: int = 128, warmup: int = 10) -> dict:
         s.set_stream(None)
     t_many = eager(sims)
     # (c) each scene replays its own captured graphs on its own stream, 64 steps per launch
-    chunk = 64
-    for s in sims:
-        s.prepare(chunk)
-    for s in sims:
-        s.synchronize()
-    t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        k = min(chunk, steps - done)
+    t_graph = None
+    if graph:
+        chunk = 64
         for s in sims:
-            s.step(k)
-        done += k
-    for s in sims:
-        s.synchronize()
-    t_graph = time.perf_counter() - t0
+            s.prepare(chunk)
+        for s in sims:
+            s.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            for s in sims:
+                s.step(k)
+            done += k
+        for s in sims:
+            s.synchronize()
+        t_graph = time.perf_counter() - t0
     for s in sims:
         s.close()
     hip.hipStreamDestroy(st)
@@ -276,10 +279,11 @@ def bench_batched(steps: int = 128, warmup: int = 10) -> dict:
             "ms_per_step": round(1e3 * tb / steps, 4),
             "per_scene_one_stream": round(total * steps / t_one, 1),
             "per_scene_stream_each": round(total * steps / t_many, 1),
-            "per_scene_graphs_stream_each": round(total * steps / t_graph, 1),
+            "per_scene_graphs_stream_each": round(total * steps / t_graph, 1) if t_graph else None,
             "speedup_vs_one_stream": round(t_one / tb, 2),
             "speedup_vs_stream_each": round(t_many / tb, 2),
-            "speedup_vs_graphs_stream_each": round(t_graph / tb, 2), "tuning": tuning}
+            "speedup_vs_graphs_stream_each": round(t_graph / tb, 2) if t_graph else None,
+            "graph": graph, "tuning": tuning}
 
 
 def bench_adjoint(nsteps: int = 2000, every: int = 100, graph: bool = True) -> dict:
@@ -358,6 +362,9 @@ def main() -> int:
                          "batched, adjoint)")
     ap.add_argument("--adjoint-graph", type=int, default=1, choices=[0, 1],
                     help="1: the adjoint extra replays hipGraphs; 0: it launches every kernel")
+    ap.add_argument("--batched-graph", type=int, default=1, choices=[0, 1],
+                    help="1: the batched extra replays hipGraphs; 0: it launches every kernel "
+                         "(and skips the per-scene graph baseline)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="finish with one plain device-to-device copy of this many MiB (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -400,7 +407,7 @@ def main() -> int:
                     extras[name] = _plain_summary(
                         bench_plain(name, k, 50, 1000), k)
                 elif name == "batched":
-                    extras[name] = bench_batched()
+                    extras[name] = bench_batched(graph=bool(args.batched_graph))
                 elif name == "adjoint":
                     extras[name] = bench_adjoint(graph=bool(args.adjoint_graph))
                 else:
