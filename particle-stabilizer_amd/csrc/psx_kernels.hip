@@ -1143,26 +1143,6 @@ __device__ __forceinline__ void pair_chunk4(const PairSelf<T>& me, const Vec4<T>
   pair_test4(me, c, vsn, q, e, self, ax, ay, az);
 }
 
-// Chunks of two candidates (c0, c1 = candidates q, q + 1 of a slice ending at e), for slices that
-// hold one or two spheres: half the prefilters of a four-wide chunk. The masks short-circuit, so
-// a candidate past its slice's end costs its lane no load and a wave whose lanes are all past it
-// skips the test; the compiler then loads c1 under c0's condition (two round trips for a chunk
-// read inside a slice). Measured on MI355X, this beats the eager form of the four-wide chunks on
-// the workloads that fill the GPU (the 4.2M-sphere hopper, the 256-scene batch), whose pair
-// kernels are bound by instruction issue and load bytes rather than by latency.
-template <class T>
-__device__ __forceinline__ void pair_test2(const PairSelf<T>& me, const Vec4<T>& c0,
-                                           const Vec4<T>& c1, const Vec4<T>* __restrict__ vsn,
-                                           uint32_t q, uint32_t e, uint32_t self, T& ax, T& ay,
-                                           T& az) {
-  const bool m0 = q < e && q != self && pair_near(me, c0);
-  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
-  if (m0 || m1) {
-    if (m0) pair_contact(me, c0, vsn, q, ax, ay, az);
-    if (m1) pair_contact(me, c1, vsn, q + 1, ax, ay, az);
-  }
-}
-
 // Slice [lo, hi) from index `from` on, the exact tests in line (a lane whose queue filled).
 template <class T>
 __device__ __forceinline__ void near_tail(const PairSelf<T>& me, const Vec4<T>* __restrict__ pos,
@@ -1205,6 +1185,13 @@ __device__ __forceinline__ void pair_chunk4q(const PairSelf<T>& me, const Vec4<T
   pair_test4q(me, c, q, e, self, nq);
 }
 
+// Chunks of two candidates (c0, c1 = candidates q, q + 1 of a slice ending at e), for slices that
+// hold one or two spheres: half the prefilters of a four-wide chunk. The masks short-circuit, so
+// a candidate past its slice's end costs its lane no load and a wave whose lanes are all past it
+// skips the test; the compiler then loads c1 under c0's condition (two round trips for a chunk
+// read inside a slice). Measured on MI355X, this beats the eager form of the four-wide chunks on
+// the workloads that fill the GPU (the 4.2M-sphere hopper, the 256-scene batch), whose pair
+// kernels are bound by instruction issue and load bytes rather than by latency.
 template <class T>
 __device__ __forceinline__ void pair_test2q(const PairSelf<T>& me, const Vec4<T>& c0,
                                             const Vec4<T>& c1, uint32_t q, uint32_t e,
