@@ -186,8 +186,14 @@ void build_tri_grid(MeshHost& m, const HostGrid& g, double rmax, bool fp32) {
             lists[(size_t)((ix * g.n[1] + iy) * g.n[2] + iz)].push_back((int32_t)t);
         }
   }
+  // the lists' offsets are int32 on the device
+  size_t entries = 0;
+  for (const auto& l : lists) entries += l.size();
+  if (entries > (size_t)INT32_MAX)
+    throw Error(PSX_ERR_INVALID, "triangle grid exceeds 2^31 cell entries (mesh too fine for the grid)");
   m.off.assign((size_t)ncell + 1, 0);
   m.list.clear();
+  m.list.reserve(entries);
   for (int64_t c = 0; c < ncell; ++c) {
     m.off[(size_t)c] = (int32_t)m.list.size();
     m.list.insert(m.list.end(), lists[(size_t)c].begin(), lists[(size_t)c].end());
@@ -455,6 +461,8 @@ struct SimImpl final : SimBase {
           tgeo.push_back(tg);
           boxes[(size_t)t] = tb;
         }
+        if (tbox.size() + m.list.size() > (size_t)INT32_MAX)
+          throw Error(PSX_ERR_INVALID, "the scenes' triangle grids exceed 2^31 cell entries in all");
         const int32_t entry_base = (int32_t)tbox.size();
         for (size_t c2 = 0; c2 + 1 < m.off.size(); ++c2) toff.push_back(entry_base + m.off[c2]);
         for (int32_t t : m.list) {
@@ -1355,7 +1363,7 @@ static int create_impl(int32_t n_scenes, const psx_scene* scenes, const double* 
   if (n_scenes <= 0 || !scenes) return set_error(PSX_ERR_INVALID, "need at least one scene");
   if (!pos || !vel || !radius || !mass) return set_error(PSX_ERR_INVALID, "particle arrays must not be NULL");
   const int32_t dtype = scenes[0].params.dtype;
-  int64_t total = 0;
+  int64_t total = 0, tris = 0;
   for (int32_t s = 0; s < n_scenes; ++s) {
     int rc = check_params(&scenes[s].params);
     if (rc) return rc;
@@ -1366,7 +1374,10 @@ static int create_impl(int32_t n_scenes, const psx_scene* scenes, const double* 
     if (scenes[s].mesh.n_triangles < 0 || (scenes[s].mesh.n_triangles > 0 && !scenes[s].mesh.vertices))
       return set_error(PSX_ERR_INVALID, "mesh vertices missing");
     total += scenes[s].n;
+    tris += scenes[s].mesh.n_triangles;
   }
+  // triangle indices are int32 on the device
+  if (tris >= ((int64_t)1 << 31) - 1) return set_error(PSX_ERR_INVALID, "too many triangles (>= 2^31)");
   if (total >= ((int64_t)1 << 31) - 1) return set_error(PSX_ERR_INVALID, "too many particles (>= 2^31)");
   for (int64_t i = 0; i < total; ++i) {
     if (!(radius[i] > 0.0) || !std::isfinite(radius[i]))

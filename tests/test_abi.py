@@ -89,6 +89,12 @@ def test_invalid_particles_rejected():
     m.vertices = None
     rc, h, err = _create(Params(), mesh=C.byref(m))
     assert rc == 1 and "mesh" in err
+    # triangle indices are int32 on the device: a count of 2^31 is refused before any vertex is read
+    one = np.zeros(9)
+    m.n_triangles = 1 << 31
+    m.vertices = one.ctypes.data_as(C.POINTER(C.c_double))
+    rc, h, err = _create(Params(), mesh=C.byref(m))
+    assert rc == 1 and "triangles" in err and not h.value
 
 
 def test_null_handles_rejected():

@@ -70,3 +70,49 @@ def test_batch_bucketing_segments(psx_mod, oracle):
             assert_bitwise(seg, ref["offset"], f"scene {k} offset")
             assert_bitwise(got["perm"][off:off + n] - off, ref["perm"], f"scene {k} perm")
             off += n
+
+
+def _crowded_scene(dtype):
+    """A box with one cell of 300 spheres and 40 cells of 9-12 (as in the overflow parity test)."""
+    rng = np.random.default_rng(11)
+    h_r = 0.5
+    prm = Params(box_min=(0.0, 0.0, 0.0), box_max=(12.0, 12.0, 12.0), dtype=dtype, dt=1e-4,
+                 spring=5.0, restitution=0.3, gravity=(0.0, 0.0, -1.0))
+    h = h_r * 2.0 * float(np.float32(1.05))
+    parts = [rng.uniform(2 * h + 0.01, 3 * h - 0.01, size=(300, 3))]
+    for _ in range(40):
+        corner = rng.integers(4, 10, size=3) * h
+        parts.append(corner + rng.uniform(0.01, h - 0.01, size=(int(rng.integers(9, 13)), 3)))
+    parts.append(rng.uniform(0.5, 11.5, size=(500, 3)))
+    pos = np.concatenate(parts)
+    rng.shuffle(pos)
+    n = pos.shape[0]
+    r = rng.uniform(0.3, h_r, n)
+    r[0] = h_r
+    vel = rng.normal(0.0, 0.1, size=(n, 3))
+    return Scene(pos, vel, r, prm, mass_of(r))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("chunk", [4, 2])
+def test_batch_crowded_scene_queue_tail(psx_mod, oracle, dtype, chunk):
+    """The crowded scene between two dense boxes in one batch: in its 300-sphere cell the pair
+    kernel's per-lane queue of near candidates fills and the lanes test the rest in line, over the
+    workgroup table. Each scene equals its plain run and the oracle bit for bit."""
+    scenes = [dense_box(n_side=5, dtype=dtype), _crowded_scene(dtype), dense_box(n_side=4, seed=3, dtype=dtype)]
+    steps = 3
+    with psx_mod.Batch(scenes) as b:
+        b.set_tuning(pair_variant=1, pair_chunk=chunk, block_pairs=128)
+        b.step(steps)
+        b.synchronize()
+        bp, bv = b.state()
+    off = 0
+    for k, sc in enumerate(scenes):
+        n = sc.radius.shape[0]
+        p, v = hip_run(sc, steps, pair_variant=1, pair_chunk=chunk, block_pairs=128)
+        assert_bitwise(bp[off:off + n], p, f"scene {k} pos vs plain")
+        assert_bitwise(bv[off:off + n], v, f"scene {k} vel vs plain")
+        rp, rv = oracle_run(oracle, sc, steps)
+        assert_bitwise(as_dtype(bp[off:off + n], dtype), rp, f"scene {k} pos vs oracle")
+        assert_bitwise(as_dtype(bv[off:off + n], dtype), rv, f"scene {k} vel vs oracle")
+        off += n
