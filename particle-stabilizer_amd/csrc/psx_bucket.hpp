@@ -8,6 +8,29 @@ namespace psx {
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
 
+// The grid cell of position x (cells numbered z-fastest).
+template <class T>
+__device__ __forceinline__ int32_t grid_cell(const SceneDev<T>& s, const Vec4<T>& x) {
+  const GridDev& g = s.grid;
+  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
+  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
+  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
+  return g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+}
+
+// Files id me into cell c's id slots at arrival slot `slot` (or onto the cell's overflow chain).
+__device__ __forceinline__ void file_id(int32_t me, int32_t c, uint32_t slot,
+                                        int32_t* __restrict__ cell_ids, uint32_t stride,
+                                        int2* __restrict__ ovf, uint32_t* __restrict__ ovf_ctr) {
+  if (slot < (uint32_t)kCellIds) {
+    cell_ids[(size_t)slot * stride + (uint32_t)c] = me;
+  } else {  // at most n entries in all
+    const uint32_t o = atomicAdd(ovf_ctr, 1u);
+    const int32_t prev = atomicExch(cell_ids + (size_t)kCellIds * stride + (uint32_t)c, (int32_t)o);
+    ovf[o] = make_int2(me, prev);
+  }
+}
+
 // ParticleModel::updateBuckets (ParticleModel.cpp:49-62) for particle p (id me) at its
 // pre-container position x: its cell and one more member counted there. With id blocks the
 // particle's arrival slot (the counter's old value) places its id in the cell's id slots, slot k of
@@ -26,28 +49,41 @@ __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4
                                                 int32_t* __restrict__ cell_ids,
                                                 uint32_t stride, int2* __restrict__ ovf,
                                                 uint32_t* __restrict__ ovf_ctr) {
-  const GridDev& g = s.grid;
-  const int32_t ix = axis_index((double)x.x, g.min[0], g.h, g.n[0]);
-  const int32_t iy = axis_index((double)x.y, g.min[1], g.h, g.n[1]);
-  const int32_t iz = axis_index((double)x.z, g.min[2], g.h, g.n[2]);
-  const int32_t c = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
+  const int32_t c = grid_cell(s, x);
   // the two branches start differently so that the compiler keeps the dense form's atomic without
   // a return value
   if (blocks) {
     const uint32_t slot = atomicAdd(count + c, 1u);
     cell_of[p] = c;
-    if (slot < (uint32_t)kCellIds) {
-      cell_ids[(size_t)slot * stride + (uint32_t)c] = me;
-    } else {  // at most n entries in all
-      const uint32_t o = atomicAdd(ovf_ctr, 1u);
-      const int32_t prev = atomicExch(cell_ids + (size_t)kCellIds * stride + (uint32_t)c, (int32_t)o);
-      ovf[o] = make_int2(me, prev);
-    }
+    file_id(me, c, slot, cell_ids, stride, ovf, ovf_ctr);
   } else {
     cell_of[p] = c;
     atomicAdd(count + c, 1u);
   }
   return c;
+}
+
+// Incremental bucketing (NextBucket::arr): particle me leaves cell c_old (c_old < 0: none, the
+// first pass) for cell c of the bucketing of generation `tag`. Both cells' arrival words are raised
+// to the generation (which marks them changed for the rank kernel and restarts a word left by an
+// earlier generation at zero arrivals), the counts move, and the particle takes the arrival slot
+// that its addition to c's word returns. The raise and the addition are atomics on one word, so
+// every addition of a generation lands on a word that already carries it.
+__device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old, uint32_t tag,
+                                            uint32_t* __restrict__ count,
+                                            unsigned long long* __restrict__ arr,
+                                            int32_t* __restrict__ cell_ids, uint32_t stride,
+                                            int2* __restrict__ ovf,
+                                            uint32_t* __restrict__ ovf_ctr) {
+  const unsigned long long t = (unsigned long long)tag << 32;
+  if (c_old >= 0) {
+    atomicSub(count + c_old, 1u);
+    atomicMax(arr + c_old, t);
+  }
+  atomicAdd(count + c, 1u);
+  atomicMax(arr + c, t);
+  const uint32_t slot = (uint32_t)atomicAdd(arr + c, 1ull);
+  file_id(me, c, slot, cell_ids, stride, ovf, ovf_ctr);
 }
 
 // Adds the calling lanes' particles (cells c) to their tiles' counts for the direct scan

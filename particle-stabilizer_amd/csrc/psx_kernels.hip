@@ -310,10 +310,22 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
 // position and the new velocity, then the bucket entry, exactly as the hash kernel computes them
 // from the state the next step starts from. Returns the cell (tile counts are added per workgroup
 // by WgTiles).
+// With the incremental bucketing only a particle whose next cell differs from its bucket cell
+// counts and files itself (bucket_move); every particle writes its next cell.
 template <class T>
 __device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextBucket& nb,
                                                Vec4<T> x, const Vec4<T>& v, int32_t p) {
   integrate_position(s, x, v);
+  if (nb.arr) {
+    const int32_t c = grid_cell(s, x);
+    nb.cell_of[p] = c;
+    const uint32_t hw = nb.home[p];
+    const int32_t c_old = (hw & kHomeMoved) ? nb.cell_cur[p] : (int32_t)(hw & kHomeCell);
+    if (c != c_old)
+      bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.cell_ids,
+                  nb.id_stride, nb.ovf, nb.ovf_ctr);
+    return c;
+  }
   return bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids,
                       nb.id_stride, nb.ovf, nb.ovf_ctr);
 }
@@ -375,7 +387,8 @@ __global__ void __launch_bounds__(1024)
                 const Vec4<T>* __restrict__ vel, const int32_t* __restrict__ id,
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
-                uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode) {
+                uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode,
+                unsigned long long* __restrict__ arr) {
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -387,7 +400,13 @@ __global__ void __launch_bounds__(1024)
         const Vec4<T> v = vel[p];
         integrate_position(s, x, v);
       }
-      c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
+      if (kBlocks && arr) {  // first pass of the incremental bucketing: every particle arrives
+        c = grid_cell(s, x);
+        cell_of[p] = c;
+        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, cell_ids, id_stride, ovf, ovf_ctr);
+      } else {
+        c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
+      }
     }
     if (tiles.tc) {
       wt.begin(c);
@@ -566,6 +585,7 @@ __global__ void __launch_bounds__(kScanThreads)
     // publish its length for the rank/gather kernel and restart the counter for the next pass
     ovf_ctr[1] = ovf_ctr[0];
     ovf_ctr[0] = 0u;
+    ovf_ctr[2] += 1u;  // the generation of this bucketing (incremental bucketing)
   }
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kScanTile);
   // Only the counts stay in registers across the look-back; the wave scans of the rounds are
@@ -674,6 +694,7 @@ __global__ void __launch_bounds__(kDirectThreads)
   if (tile == 0 && threadIdx.x == 0) {  // as in the look-back scan
     ovf_ctr[1] = ovf_ctr[0];
     ovf_ctr[0] = 0u;
+    ovf_ctr[2] += 1u;  // the generation of this bucketing (incremental bucketing)
   }
   uint4* __restrict__ src = reinterpret_cast<uint4*>(count + (size_t)tile * kDirectTile);
   const uint4 u = src[threadIdx.x];
@@ -732,6 +753,39 @@ __global__ void __launch_bounds__(kDirectThreads)
 // members, from the cell's chain of overflow entries (count - kCellIds of them), so the cost of a
 // particle is bounded by its own cell's occupancy.
 // ---------------------------------------------------------------------------------------------
+// The rank of particle me (old sorted slot p, bucket cell c) in the incremental bucketing (IncRank):
+// its place in the cell's old range when the cell's arrival word is of an earlier generation
+// (nothing entered or left it), else the members of the old range that stay in c and the cell's
+// arrivals (id slots, then the overflow chain), counted where their id is smaller. The range is
+// clamped to the particles, so stale offsets cannot send the loop past the arrays.
+__device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, int32_t c, int32_t me,
+                                             uint32_t p, uint32_t n,
+                                             const int32_t* __restrict__ cell_of,
+                                             const int32_t* __restrict__ id,
+                                             const int32_t* __restrict__ cell_ids,
+                                             uint32_t id_stride, const int2* __restrict__ ovf) {
+  const uint64_t w = inc.arr[c];
+  const uint32_t lo = inc.off_old[c];
+  if ((uint32_t)(w >> 32) != gen) return p - lo;
+  const uint32_t hi = min(inc.off_old[c + 1], n);
+  uint32_t rank = 0;
+  for (uint32_t q = min(lo, hi); q < hi; ++q)
+    rank += (cell_of[q] == c && id[q] < me) ? 1u : 0u;
+  const uint32_t na = (uint32_t)w;
+#pragma unroll
+  for (int k = 0; k < kCellIds; ++k)
+    if ((uint32_t)k < na) rank += (cell_ids[(size_t)k * id_stride + (uint32_t)c] < me) ? 1u : 0u;
+  if (na > (uint32_t)kCellIds) {  // the arrivals past the slots, newest first
+    int32_t o = cell_ids[(size_t)kCellIds * id_stride + (uint32_t)c];
+    for (uint32_t k = kCellIds; k < na; ++k) {
+      const int2 q = ovf[o];
+      rank += (q.x < me) ? 1u : 0u;
+      o = q.y;
+    }
+  }
+  return rank;
+}
+
 template <class T, bool kTable, bool kBlocks>
 __global__ void __launch_bounds__(kMaxBlockSort<T>)
     k_rank_gather(const Locator<T> loc, const int32_t* __restrict__ cell_of,
@@ -745,9 +799,10 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
                   const uint8_t* __restrict__ cull, int32_t n, Vec4<T>* __restrict__ pos_o,
                   Vec4<T>* __restrict__ vel_o, int32_t* __restrict__ id_o,
                   int32_t* __restrict__ cell_o, uint32_t* __restrict__ home_o,
-                  Vec4<T>* __restrict__ xpre_o, uint32_t* __restrict__ gen) {
+                  Vec4<T>* __restrict__ xpre_o, uint32_t* __restrict__ gen, const IncRank inc) {
   // the scan of this bucketing is complete: the next hash pass counts tiles for the next one
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(gen, 1u);
+  const uint32_t inc_gen = inc.arr ? ovf_ctr[2] : 0u;  // generation of this bucketing
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     const bool active = p < p1;  // inactive lanes stay for the wave-wide mesh path
@@ -776,6 +831,10 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       const uint32_t b = offset[c], e = offset[c + 1];
       uint32_t rank = 0;
       if constexpr (kBlocks) {
+        if (inc.arr) {
+          rank = inc_rank(inc, inc_gen, c, me, (uint32_t)p, (uint32_t)n, cell_of, id, cell_ids,
+                          id_stride, ovf);
+        } else {
         const uint32_t cnt = e - b;
         if (cnt > 1u) {  // a particle alone in its cell has rank 0
 #pragma unroll
@@ -789,6 +848,7 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
             rank += (q.x < me) ? 1u : 0u;
             o = q.y;
           }
+        }
         }
       } else {  // the first four members in one go (the list carries kListPad spare entries)
         const int32_t l0 = list[b], l1 = list[b + 1], l2 = list[b + 2], l3 = list[b + 3];
@@ -820,7 +880,9 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       home = g.cell_base + (ix * g.n[1] + iy) * g.n[2] + iz;
       cull_c = cull[home];
     }
-    home_o[dst] = (uint32_t)home | ((cull_c & kCellCull) ? kHomeCull : 0u);
+    const bool moved = inc.cell_cur_o && home != c;  // incremental bucketing: see kHomeMoved
+    home_o[dst] = (uint32_t)home | ((cull_c & kCellCull) ? kHomeCull : 0u) | (moved ? kHomeMoved : 0u);
+    if (moved) inc.cell_cur_o[dst] = c;
     if (xpre_o) xpre_o[dst] = xpre;
     if (mode & kModeCheck) {
       const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
@@ -946,7 +1008,7 @@ __device__ __forceinline__ void pair_setup(const SceneDev<T>* __restrict__ s,
   // home cell of the snapshot position (ParticleModel.cpp:106-110, clamped) and its cull flag,
   // from the gather
   const uint32_t hw = home_of[p];
-  const int32_t home = (int32_t)(hw & ~kHomeCull);
+  const int32_t home = (int32_t)(hw & kHomeCell);
   const T rr = me.x.w + s->rmax;
   me.thr = (rr * rr) * PairMargin<T>::value;
   me.ks = s->spring;
@@ -1402,7 +1464,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     const bool tiles = next.count && next.tiles.tc;
-    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    if (tiles) wt.begin((int32_t)(home[p0] & kHomeCell));
     int32_t c = -1;
     if (p < p1) {
       PairSelf<T> me;
@@ -1412,7 +1474,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
         me.x = pos[p];
         me.v = vsn[p];
         const uint32_t hw = home[p];
-        const int32_t hc = (int32_t)(hw & ~kHomeCull);
+        const int32_t hc = (int32_t)(hw & kHomeCell);
         const T rr = me.x.w + sc.rmax;
         me.thr = (rr * rr) * PairMargin<T>::value;
         me.ks = sc.spring;
@@ -1479,7 +1541,7 @@ __global__ void __launch_bounds__(1024)
     const int32_t g = (int32_t)threadIdx.x % G;
     const int32_t p = p0 + (int32_t)threadIdx.x / G;
     const bool tiles = next.count && next.tiles.tc;
-    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    if (tiles) wt.begin((int32_t)(home[p0] & kHomeCell));
     int32_t c = -1;
     if (p < p1) {  // whole groups take part or not
       PairSelf<T> me;
@@ -1542,11 +1604,11 @@ __global__ void __launch_bounds__(512)
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& sc, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
     const bool tiles = next.count && next.tiles.tc;
-    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    if (tiles) wt.begin((int32_t)(home[p0] & kHomeCell));
     if (threadIdx.x < 9) {
       const int d = (int)threadIdx.x;
       const GridDev& g = sc.grid;
-      const int32_t h0 = (int32_t)(home[p0] & ~kHomeCull), h1 = (int32_t)(home[p1 - 1] & ~kHomeCull);
+      const int32_t h0 = (int32_t)(home[p0] & kHomeCell), h1 = (int32_t)(home[p1 - 1] & kHomeCell);
       const int32_t col = ((d / 3 - 1) * g.n[1] + (d % 3 - 1)) * g.n[2];
       const int32_t first = g.cell_base, last = g.cell_base + g.n[0] * g.n[1] * g.n[2] - 1;
       const int32_t cs = max(min(h0, h1) + col - 1, first), ce = min(max(h0, h1) + col + 1, last);
@@ -1823,12 +1885,14 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode) {
+                            const TileCounts& tiles, int mode, uint64_t* arr) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
+  if (arr && (!blocks || tiles.tc)) return hipErrorInvalidValue;  // id slots, look-back scan only
+  unsigned long long* const arr_w = reinterpret_cast<unsigned long long*>(arr);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode)
+             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1908,14 +1972,15 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
                               int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o,
-                              uint32_t* gen) {
+                              uint32_t* gen, const IncRank& inc) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
+  if (inc.arr && (!blocks || !inc.off_old || !inc.cell_cur_o)) return hipErrorInvalidValue;
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
              cell_ids, id_stride, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode,\
              cull, n,                                                                          \
-             pos_o, vel_o, id_o, cell_o, home_o, xpre_o, gen)
+             pos_o, vel_o, id_o, cell_o, home_o, xpre_o, gen, inc)
   if (one) {
     if (blocks) PSX_RANK(false, true);
     else PSX_RANK(false, false);
@@ -1978,7 +2043,7 @@ __global__ void __launch_bounds__(1024)
     Vec4<T>* const win =
         reinterpret_cast<Vec4<T>*>(wave_raw) + (size_t)(threadIdx.x >> 6) * (2 * kWaveWin);
     const bool tiles = next.count && next.tiles.tc;
-    if (tiles) wt.begin((int32_t)(home[p0] & ~kHomeCull));
+    if (tiles) wt.begin((int32_t)(home[p0] & kHomeCell));
     const bool active = p < p1;  // the active lanes are a prefix of the wave
     const GridDev& g = sc.grid;
     int32_t c = -1;
@@ -1991,7 +2056,7 @@ __global__ void __launch_bounds__(1024)
       for (int d = 0; d < 9; ++d) lo[d] = hi[d] = 0;
       if (active) {
         pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
-        cell_coords(g, (int32_t)(home[p] & ~kHomeCull), hx, hy, hz);
+        cell_coords(g, (int32_t)(home[p] & kHomeCell), hx, hy, hz);
         col = hx * g.n[1] + hy;
       }
       // the wave's first home column (lane 0's) and the first other one; the first and last lane
@@ -2146,7 +2211,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
-                                          const TileCounts&, int);                                     \
+                                          const TileCounts&, int, uint64_t*);                          \
   template hipError_t launch_front<T>(hipStream_t, int, int, const BlockInfo*,                  \
                                       const SceneDev<T>*, const SceneDev<T>*, bool, Vec4<T>*,  \
                                       Vec4<T>*, const int32_t*, const TriEntry<T>*,            \
@@ -2160,7 +2225,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                             const int32_t*, const TriEntry<T>*,                \
                                             const int32_t*, uint32_t*, int, const uint8_t*,    \
                                             Vec4<T>*, Vec4<T>*, int32_t*, int32_t*,            \
-                                            uint32_t*, Vec4<T>*, uint32_t*);                              \
+                                            uint32_t*, Vec4<T>*, uint32_t*, const IncRank&);              \
   template hipError_t launch_pairs<T>(hipStream_t, int, int, int, int, const BlockInfo*,            \
                                       const SceneDev<T>*, const SceneDev<T>*, const Vec4<T>*,  \
                                       const Vec4<T>*, const uint32_t*, const uint32_t*,        \

@@ -23,6 +23,11 @@ constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
 constexpr int64_t kIdPlane = 4096;
 constexpr int kCellIds = 8;  // id slots per cell; arrivals beyond go to the cell's overflow chain
 constexpr uint32_t kHomeCull = 0x80000000u;  // home word: the home cell may cull its neighbours
+// Home word of the incremental bucketing: the container moved the particle out of its bucket
+// cell (the bucket cell is then in the cell_cur array at the particle's sorted slot). Grids hold at
+// most 2^30 cells, so the cell index fits below both flag bits.
+constexpr uint32_t kHomeMoved = 0x40000000u;
+constexpr uint32_t kHomeCell = 0x3fffffffu;
 // Per-cell flag byte (static per scene): the cell's 27 neighbours may be culled by distance in the
 // pair kernel; the cell's triangle list is not empty (only such cells' lists are read).
 constexpr uint8_t kCellCull = 1, kCellTris = 2;
@@ -73,6 +78,15 @@ struct TileCounts {
   int32_t ntiles;
 };
 
+// Incremental bucketing (arr != nullptr; look-back-scan grids, id slots): the counts persist from
+// one step to the next and only a particle whose next cell differs from its bucket cell touches
+// them (one less in the old cell, one more in the new). arr holds one word per cell,
+// (generation << 32) | arrivals: a particle that leaves or enters a cell raises the word to the
+// generation of the coming bucketing (ovf_ctr[2] + 1; the scan advances ovf_ctr[2] to it), an
+// entering one then adds itself to the arrivals and takes the id slot that the sum returns. The rank
+// kernel finds a cell that nothing entered or left by its stale generation. The bucket cell of the
+// particle at sorted slot p is its home cell (home[p] & kHomeCell) unless kHomeMoved is set, then
+// cell_cur[p].
 struct NextBucket {
   const int32_t* id;
   int32_t* cell_of;
@@ -83,6 +97,9 @@ struct NextBucket {
   uint32_t* ovf_ctr;
   int32_t blocks;
   TileCounts tiles;
+  unsigned long long* arr;  // incremental bucketing: per-cell arrival words (nullptr: full)
+  const uint32_t* home;     // incremental bucketing: home words of the pair kernel's state
+  const int32_t* cell_cur;  // incremental bucketing: bucket cells where kHomeMoved is set
 };
 
 // Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
@@ -95,13 +112,15 @@ struct NextBucket {
 // kIdPlane, then one plane of per-cell overflow chain heads; ovf: n entries (id, previous entry of
 // the cell) for arrivals past kCellIds; ovf_ctr[0] counts them and must be 0 on entry, k_scan
 // resets it). Otherwise launch_scatter_ids builds dense cell lists once the offsets are known.
+// arr != nullptr: the first pass of the incremental bucketing (NextBucket): every particle enters
+// its cell as an arrival (counts zero on entry, blocks set).
 template <class T>
 hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo* btab,
                             const SceneDev<T>* scenes, const SceneDev<T>* one, bool blocks,
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode);
+                            const TileCounts& tiles, int mode, uint64_t* arr = nullptr);
 
 // The first kernel of a step with PSX_FLAG_SEMANTICS_CUDA: integration, the container (box walls or
 // the CUDA backend's mesh contact), the state written back to pos/vel, then the hash pass of the
@@ -125,8 +144,8 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
                               int32_t* list);
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
-// (the last entry is the total); clears the counts when `clear`, moves ovf_ctr[0] to ovf_ctr[1]
-// and zeroes ovf_ctr[0]. tile_state (ntiles words) persists between launches and is zeroed once at
+// (the last entry is the total); clears the counts when `clear`, moves ovf_ctr[0] to ovf_ctr[1],
+// zeroes ovf_ctr[0] and advances the generation ovf_ctr[2] (ovf_ctr holds three words). tile_state (ntiles words) persists between launches and is zeroed once at
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear);
@@ -135,6 +154,18 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 // per-tile counts of the hash pass (TileCounts); no tile waits for another.
 hipError_t launch_scan_direct(hipStream_t st, uint32_t* count, uint32_t* offset,
                               const TileCounts& tiles, uint32_t* ovf_ctr, bool clear);
+
+// The rank of the incremental bucketing (arr != nullptr; see NextBucket): off_old holds the offsets
+// of the state being ranked (its sorted slot p lies in off_old[c] .. off_old[c + 1] of its bucket
+// cell c). A particle whose cell nothing entered or left keeps its place in that range; in any other
+// cell it counts the cell's remaining members with a smaller id (old members whose new cell is still
+// the cell) and its arrivals with a smaller id. cell_cur_o receives the bucket cell of every sorted
+// slot whose home cell differs from it (kHomeMoved).
+struct IncRank {
+  const uint64_t* arr = nullptr;
+  const uint32_t* off_old = nullptr;
+  int32_t* cell_cur_o = nullptr;
+};
 
 // Stable rank among cell mates (from the id blocks, or from the dense lists when !blocks); the
 // integration step (kModeIntegrate), container (kModeContainer:
@@ -154,7 +185,8 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               const int32_t* id, const TriEntry<T>* tent,
                               const int32_t* tcell_off, uint32_t* flags, int mode,
                               const uint8_t* cull, Vec4<T>* pos_o, Vec4<T>* vel_o, int32_t* id_o,
-                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o, uint32_t* gen);
+                              int32_t* cell_o, uint32_t* home_o, Vec4<T>* xpre_o, uint32_t* gen,
+                              const IncRank& inc = IncRank{});
 
 // The pair kernel form that stages the workgroup's neighbour particles in LDS (pair_variant value;
 // one lane per particle): at most kStageFactor sorted positions per workgroup thread.

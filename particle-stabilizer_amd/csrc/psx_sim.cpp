@@ -293,7 +293,11 @@ struct SimImpl final : SimBase {
   DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, list, stat_part;
   DevBuf tile_count, scan_gen;  // direct scan: per-tile counts (two generations), generation
   int32_t ntiles_direct = 0;    // tiles of the direct scan, 0: look-back scan
+  int32_t ntiles_direct_grid = 0;  // tiles of the direct scan when the grid and the form allow it
   DevBuf count, offset, tile_state, flags;
+  // incremental bucketing (sort_variant 3): per-cell arrival words, the second offset array (the
+  // offsets alternate with the position buffers) and the bucket cells of container-moved spheres
+  DevBuf arr, offset2, cell_cur;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
@@ -484,7 +488,8 @@ struct SimImpl final : SimBase {
     cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
     ntiles = (int32_t)(cpad / kScanTile);
     id_stride = (cells + kIdPlane - 1) / kIdPlane * kIdPlane;
-    ntiles_direct = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
+    ntiles_direct_grid = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
+    ntiles_direct = incremental() ? 0 : ntiles_direct_grid;
 
     // device allocations
     for (int k = 0; k < 2; ++k) {
@@ -497,11 +502,11 @@ struct SimImpl final : SimBase {
     list.alloc(((size_t)nn + kListPad) * 4);
     cell_ids.alloc((size_t)id_stride * (kCellIds + 1) * 4);  // id slots + overflow chain heads
     ovf.alloc((size_t)nn * 8);
-    ovf_ctr.alloc(8);
+    ovf_ctr.alloc(16);  // overflow entries, their published number, the bucketing generation
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1 + kOffsetPad) * 4);
     tile_state.alloc((size_t)ntiles * 8);
-    tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct * kTileStride, 1) * 4);
+    tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct_grid * kTileStride, 1) * 4);
     scan_gen.alloc(4);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
@@ -543,7 +548,33 @@ struct SimImpl final : SimBase {
     table(tuning.block_sort, &nb);
     table(pair_particles_per_block(), &nb);
     table(kStatBlock, &nb);
+    inc_buffers();
     PSX_HIP(hipDeviceSynchronize());
+  }
+
+  // Incremental bucketing (sort_variant 3): kept counts, atomics only for spheres that change cell
+  // (NextBucket, IncRank). It takes the look-back scan on every grid (the direct scan's per-tile
+  // counts are rebuilt by every hash pass); the CUDA backend's step uses the id slots of
+  // sort_variant 1.
+  bool incremental() const { return tuning.sort_variant == 3 && !cuda; }
+
+  // The buffers of the incremental bucketing, allocated when the tuning asks for it (at set-up or
+  // retune, outside any step or capture) and zeroed: arrival words of generation 0 are older than
+  // every bucketing.
+  void inc_buffers() {
+    if (!incremental() || arr.p) return;
+    arr.alloc((size_t)cpad * 8);
+    offset2.alloc(offset.bytes);
+    cell_cur.alloc((size_t)n * 4);
+    PSX_HIP(hipMemset(arr.p, 0, arr.bytes));
+    PSX_HIP(hipMemset(offset2.p, 0, offset2.bytes));
+    PSX_HIP(hipDeviceSynchronize());
+  }
+
+  // Offsets of the incremental bucketing's state in position buffer k (k = 0: the offsets every
+  // other path uses).
+  uint32_t* offs_at(int k) const {
+    return k == 0 ? offs() : offset2.as<uint32_t>() + kOffsetLead;
   }
 
   // One kernel launch, optionally bracketed by timing events.
@@ -583,7 +614,9 @@ struct SimImpl final : SimBase {
   }
 
   // Hash pass: cell, count and id-block entry or slot of every particle (counts must be zero).
-  void hash(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, int mode, Profiler* prof) {
+  // inc: the first pass of the incremental bucketing (every sphere arrives in its cell).
+  void hash(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, int mode, Profiler* prof,
+            bool inc = false) {
     int nb_i;
     const BlockInfo* bt_i = table(tuning.block_integrate, &nb_i);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
@@ -593,25 +626,33 @@ struct SimImpl final : SimBase {
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
-                                 tiles(), mode);
+                                 tiles(), mode, inc ? arr.as<uint64_t>() : nullptr);
     });
   }
 
   // Ordering after a hash pass: offsets (clearing the counts), lists, rank/gather into pd/vd/idd.
+  // off_old: the incremental bucketing's order, from the offsets of the state (ps, vs, is); its
+  // counts are kept.
   void order(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
              int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
-             Vec4<T>* xpre_sorted) {
+             Vec4<T>* xpre_sorted, const uint32_t* off_old = nullptr) {
     int nb_s;
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
     const int32_t nn = (int32_t)n;
     const bool blocks = id_blocks();
+    IncRank inc;
+    if (off_old) {
+      inc.arr = arr.as<uint64_t>();
+      inc.off_old = off_old;
+      inc.cell_cur_o = cell_cur.as<int32_t>();
+    }
     launch(1, prof, [&] {
       if (ntiles_direct)
         return launch_scan_direct(stream, count.as<uint32_t>(), off, tiles(), ovf_ctr.as<uint32_t>(),
                                   blocks);
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
-                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks);
+                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks && !off_old);
     });
     if (!blocks)
       launch(4, prof, [&] {
@@ -626,7 +667,7 @@ struct SimImpl final : SimBase {
                                    vs, is, tent_d.as<TriEntry<T>>(), tcell_off_d.as<int32_t>(),
                                    flags.as<uint32_t>(), mode, cull_d.as<uint8_t>(), pd, vd, idd,
                                    cell_sorted, home.as<uint32_t>(), xpre_sorted,
-                                   scan_gen.as<uint32_t>());
+                                   scan_gen.as<uint32_t>(), inc);
     });
   }
 
@@ -654,8 +695,8 @@ struct SimImpl final : SimBase {
   }
 
   // Arguments of a hash pass done inside another kernel, for the particles with ids `ids` (in the
-  // kernel's particle order); ids == nullptr: none.
-  NextBucket next_bucket_args(const int32_t* ids) const {
+  // kernel's particle order); ids == nullptr: none. inc: the incremental bucketing's.
+  NextBucket next_bucket_args(const int32_t* ids, bool inc = false) const {
     NextBucket nb;
     std::memset(&nb, 0, sizeof(nb));
     if (ids) {
@@ -668,6 +709,11 @@ struct SimImpl final : SimBase {
       nb.ovf_ctr = ovf_ctr.as<uint32_t>();
       nb.blocks = id_blocks() ? 1 : 0;
       nb.tiles = tiles();
+      if (inc) {
+        nb.arr = arr.as<unsigned long long>();
+        nb.home = home.as<uint32_t>();
+        nb.cell_cur = cell_cur.as<int32_t>();
+      }
     }
     return nb;
   }
@@ -675,10 +721,10 @@ struct SimImpl final : SimBase {
   // next_ids: with the sorted ids of the pair kernel's state, it also does the next step's hash
   // pass (NextBucket); nullptr: velocities only.
   void pairs(const Vec4<T>* ps, const Vec4<T>* vs, const uint32_t* off, Vec4<T>* vout,
-             Profiler* prof, const int32_t* next_ids = nullptr) {
+             Profiler* prof, const int32_t* next_ids = nullptr, bool inc = false) {
     int nb_p;
     const BlockInfo* bt_p = table(pair_particles_per_block(), &nb_p);
-    const NextBucket nb = next_bucket_args(next_ids);
+    const NextBucket nb = next_bucket_args(next_ids, inc);
     launch(3, prof, [&] {
       return launch_pairs<T>(stream, nb_p, tuning.block_pairs, pair_lanes(),
                              tuning.pair_chunk == 2 ? 2 : 4, single() ? nullptr : bt_p, scenes_d.as<SceneDev<T>>(),
@@ -691,10 +737,15 @@ struct SimImpl final : SimBase {
   // kernel): the counts hold it and the next step skips k_integrate.
   bool hashed = false;
 
+  // The incremental bucketing starts with every sphere arriving in its cell and with empty ranges
+  // in the offsets of the state (no sphere stays): those offsets are zeroed.
   void ensure_hashed(Profiler* prof) {
     if (hashed || cuda) return;  // the CUDA backend's step hashes after its container
+    const bool inc = incremental();
+    if (inc)
+      PSX_HIP(launch_zero_u32(stream, offs_at(cur_pos) - kOffsetLead, (int64_t)(offset.bytes / 4)));
     hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
-         kModeIntegrate, prof);
+         kModeIntegrate, prof, inc);
     hashed = true;
   }
 
@@ -705,7 +756,7 @@ struct SimImpl final : SimBase {
     if (!hashed) return;
     PSX_HIP(launch_zero_u32(stream, count.as<uint32_t>(), (int64_t)cpad));
     PSX_HIP(launch_zero_u32(stream, ovf_ctr.as<uint32_t>(), 1));
-    PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct * kTileStride));
+    PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct_grid * kTileStride));
     hashed = false;
   }
 
@@ -741,11 +792,15 @@ struct SimImpl final : SimBase {
     }
     const int a = cur_pos, vc = cur_vel;
     ensure_hashed(prof);
+    const bool inc = incremental();
+    // the incremental bucketing reads the offsets of the state it orders (position buffer a) and
+    // writes those of the sorted state into the other offset array
+    uint32_t* const off = inc ? offs_at(1 - a) : offs();
     order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
-          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), offs(), nullptr,
-          kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr);
-    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), offs(),
-          vel[vc].as<Vec4<T>>(), prof, id[1 - a].as<int32_t>());
+          vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), off, nullptr,
+          kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr, inc ? offs_at(a) : nullptr);
+    pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), off,
+          vel[vc].as<Vec4<T>>(), prof, id[1 - a].as<int32_t>(), inc);
     cur_pos = 1 - a;
   }
 
@@ -1099,10 +1154,17 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipStreamSynchronize(stream));
     destroy_graphs();
     tuning = t;
+    const int32_t was = ntiles_direct;
+    ntiles_direct = incremental() ? 0 : ntiles_direct_grid;
+    if (ntiles_direct != was) {  // the per-tile counts of both generations start again from zero
+      PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct_grid * kTileStride));
+      PSX_HIP(hipStreamSynchronize(stream));
+    }
     int nb;
     table(tuning.block_integrate, &nb);
     table(tuning.block_sort, &nb);
     table(pair_particles_per_block(), &nb);
+    inc_buffers();
   }
 
   void get_state(double* hp, double* hv) override {
@@ -1514,8 +1576,9 @@ int psx_set_tuning(psx_sim* sim, const psx_tuning* t) {
   if (c.use_graph != 0 && c.use_graph != 1) return set_error(PSX_ERR_INVALID, "use_graph must be 0 or 1");
   if (sim->impl->dtype() == PSX_F64 && c.block_sort > psx::kMaxBlockSort<double>)
     return set_error(PSX_ERR_INVALID, "block_sort must be at most 512 for a binary64 simulation");
-  if (c.sort_variant < 0 || c.sort_variant > 2)
-    return set_error(PSX_ERR_INVALID, "sort_variant must be 0 (automatic), 1 (id blocks) or 2 (dense lists)");
+  if (c.sort_variant < 0 || c.sort_variant > 3)
+    return set_error(PSX_ERR_INVALID, "sort_variant must be 0 (automatic), 1 (id blocks), 2 (dense lists) "
+                                      "or 3 (incremental id blocks)");
   if (c.pair_variant != 0 && c.pair_variant != 1 && c.pair_variant != 2 && c.pair_variant != 4 &&
       c.pair_variant != 8 && c.pair_variant != 16 && c.pair_variant != psx::kPairStaged &&
       c.pair_variant != psx::kPairWave)

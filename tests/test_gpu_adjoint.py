@@ -19,10 +19,12 @@ def _weights(n, seed=3):
     return rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
 
 
-def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10, graph=False):
+def _hip(psx_mod, sc, wx, wv, steps=STEPS, every=10, graph=False, **tuning):
     sim = psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh)
     if graph:
         sim.set_tuning(use_graph=1)
+    if tuning:
+        sim.set_tuning(**tuning)
     dt = torch.float64 if sc.params.dtype == "f64" else torch.float32
     x0 = torch.tensor(sc.pos, dtype=dt, device="cuda", requires_grad=True)
     v0 = torch.tensor(sc.vel, dtype=dt, device="cuda", requires_grad=True)

@@ -23,6 +23,7 @@ def _cuda(sc):
 SHAPES = [dict(),
           dict(block_pairs=128, block_integrate=64, pair_variant=2, sort_variant=2),
           dict(block_pairs=256, block_sort=128, pair_chunk=2, sort_variant=1),
+          dict(block_pairs=256, pair_chunk=2, sort_variant=3),  # the CUDA step takes id slots
           dict(block_pairs=128, block_integrate=512, pair_variant=32, sort_variant=1)]
 
 

@@ -14,7 +14,9 @@ pytestmark = pytest.mark.gpu
 
 def _bucket_compare(oracle, sim, pos, dtype, rmax, mn, mx):
     ref = oracle.bucketing(rmax, mn, mx, as_dtype(pos, dtype), dtype=dtype)
-    for variant in (1, 2):  # per-cell id blocks, dense lists
+    # per-cell id blocks, dense lists, and the full bucketing of the incremental form's handles
+    # (the look-back scan on every grid)
+    for variant in (1, 2, 3):
         sim.set_tuning(sort_variant=variant)
         got = sim.bucketing()
         for k in ("cell", "count", "offset", "perm"):
@@ -228,7 +230,7 @@ def test_tuning_rejects_bad_lanes(psx_mod):
         with pytest.raises(psx_mod.PsxError, match="512"):
             s.set_tuning(block_sort=1024)  # binary64 rank/gather: at most 512
         s.set_tuning(block_sort=512)
-        for bad in (-1, 3):
+        for bad in (-1, 4):
             with pytest.raises(psx_mod.PsxError, match="sort_variant"):
                 s.set_tuning(sort_variant=bad)
 
