@@ -44,19 +44,24 @@ def tuned(sim) -> dict:
     return autotune(sim, cache=TUNE_CACHE, force=RETUNE)["tuning"]
 
 
-def algorithmic_bytes(n: int, cells: int, s: int, blocks: bool) -> dict:
-    """Bytes each kernel must move per launch (each array touched once; s = scalar bytes; blocks:
-    the per-cell id-slot form of the cell lists, else the dense lists). In a run of steps the pair
-    kernel also does the next step's hash pass: ids read, cells written, ids filed into the id
-    slots (4 N each) and one count read and written per sphere (8 N); k_integrate only starts a
-    run. The pair kick itself reads the sorted state (8 s N) and the cell offsets (8 C) and writes
-    the new velocities (3 s N)."""
+def algorithmic_bytes(n: int, cells: int, s: int, form: int) -> dict:
+    """Bytes each kernel must move per launch (each array touched once; s = scalar bytes; form:
+    the cell-list form, sort_variant 1 or 0 per-cell id slots, 2 dense lists, 3 incremental id
+    slots). In a run of steps the pair kernel also does the next step's hash pass: ids read, cells
+    written, ids filed into the id slots (4 N each) and one count read and written per sphere
+    (8 N); in the incremental form only the cells are written (4 N), the spheres that change cell
+    (a few per thousand in a settle) being left out; k_integrate only starts a run. The pair kick
+    itself reads the sorted state (8 s N) and the cell offsets (8 C) and writes the new velocities
+    (3 s N). The incremental form's scan leaves the counts alone (8 C instead of 12 C) and its rank
+    kernel reads each sphere's cell mark and old offset (5 N more)."""
+    blocks = form != 2
+    inc = form == 3
     return {
         "k_integrate": n * (13 * s + 16),
-        "k_scan": (12 if blocks else 8) * cells,
+        "k_scan": (12 if blocks and not inc else 8) * cells,
         "k_scatter_ids": 0 if blocks else 16 * n + 8 * cells,
-        "k_rank_gather": n * (16 * s + 24),
-        "k_pairs": 11 * s * n + 8 * cells + n * (12 if blocks else 8) + 8 * n,
+        "k_rank_gather": n * (16 * s + 24 + (5 if inc else 0)),
+        "k_pairs": 11 * s * n + 8 * cells + (4 * n if inc else n * (12 if blocks else 8) + 8 * n),
     }
 
 
@@ -173,7 +178,7 @@ def bench_plain(workload: str, steps: int, warmup: int, profile_steps: int, dtyp
     psteps = max(1, min(profile_steps, steps))
     prof = sim.profile(psteps)
     cells = sim.num_cells
-    algo = algorithmic_bytes(n, cells, s_bytes, tuning.get("sort_variant", 0) != 2)
+    algo = algorithmic_bytes(n, cells, s_bytes, tuning.get("sort_variant", 0))
     kernels = {}
     for name, d in prof.items():
         if d["per_step"] == 0:

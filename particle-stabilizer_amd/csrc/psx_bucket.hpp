@@ -68,10 +68,12 @@ __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4
 // to the generation (which marks them changed for the rank kernel and restarts a word left by an
 // earlier generation at zero arrivals), the counts move, and the particle takes the arrival slot
 // that its addition to c's word returns. The raise and the addition are atomics on one word, so
-// every addition of a generation lands on a word that already carries it.
+// every addition of a generation lands on a word that already carries it. Both cells' marks take
+// the generation's low byte (plain stores: every writer of a generation stores the same value).
 __device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old, uint32_t tag,
                                             uint32_t* __restrict__ count,
                                             unsigned long long* __restrict__ arr,
+                                            uint8_t* __restrict__ mark,
                                             int32_t* __restrict__ cell_ids, uint32_t stride,
                                             int2* __restrict__ ovf,
                                             uint32_t* __restrict__ ovf_ctr) {
@@ -79,9 +81,11 @@ __device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old
   if (c_old >= 0) {
     atomicSub(count + c_old, 1u);
     atomicMax(arr + c_old, t);
+    mark[c_old] = (uint8_t)tag;
   }
   atomicAdd(count + c, 1u);
   atomicMax(arr + c, t);
+  mark[c] = (uint8_t)tag;
   const uint32_t slot = (uint32_t)atomicAdd(arr + c, 1ull);
   file_id(me, c, slot, cell_ids, stride, ovf, ovf_ctr);
 }

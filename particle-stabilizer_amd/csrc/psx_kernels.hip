@@ -322,7 +322,7 @@ __device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextB
     const uint32_t hw = nb.home[p];
     const int32_t c_old = (hw & kHomeMoved) ? nb.cell_cur[p] : (int32_t)(hw & kHomeCell);
     if (c != c_old)
-      bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.cell_ids,
+      bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.mark, nb.cell_ids,
                   nb.id_stride, nb.ovf, nb.ovf_ctr);
     return c;
   }
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(1024)
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
                 uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode,
-                unsigned long long* __restrict__ arr) {
+                unsigned long long* __restrict__ arr, uint8_t* __restrict__ mark) {
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(1024)
       if (kBlocks && arr) {  // first pass of the incremental bucketing: every particle arrives
         c = grid_cell(s, x);
         cell_of[p] = c;
-        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, cell_ids, id_stride, ovf, ovf_ctr);
+        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, mark, cell_ids, id_stride, ovf, ovf_ctr);
       } else {
         c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
       }
@@ -567,7 +567,7 @@ __device__ inline uint64_t tile_word(uint32_t value, uint64_t status, uint32_t e
 __global__ void __launch_bounds__(kScanThreads)
     k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
            uint64_t* __restrict__ tile_state, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
-           uint32_t* __restrict__ flags, int32_t clear) {
+           uint32_t* __restrict__ flags, int32_t clear, uint32_t limit) {
   // A tile is kScanRounds sub-tiles of kScanThreads uint4 (4 cells each): thread t holds uint4
   // t of every sub-tile, so every load and store of a wave is contiguous.
   constexpr int kScanRounds = kScanPerThread / 4;
@@ -662,19 +662,22 @@ __global__ void __launch_bounds__(kScanThreads)
       rtot += wsum[q][k];
     }
     uint4 o;
-    o.x = run;
+    o.x = min(run, limit);
     run += u[q].x;
-    o.y = run;
+    o.y = min(run, limit);
     run += u[q].y;
-    o.z = run;
+    o.z = min(run, limit);
     run += u[q].z;
-    o.w = run;
+    o.w = min(run, limit);
     dst[q * kScanThreads + threadIdx.x] = o;
     if (clear && (u[q].x | u[q].y | u[q].z | u[q].w) != 0u) src[q * kScanThreads + threadIdx.x] = zero;
     round_base += rtot;
   }
-  if (tile == ntiles - 1 && threadIdx.x == 0)
-    offset[(size_t)ntiles * kScanTile] = round_base;  // total: the offset one past the last cell
+  if (tile == ntiles - 1 && threadIdx.x == 0) {
+    // total: the offset one past the last cell
+    offset[(size_t)ntiles * kScanTile] = min(round_base, limit);
+    if (round_base != limit) atomicOr(flags, kFlagCountDrift);
+  }
 }
 
 // The direct scan (TileCounts): tile b's prefix is the sum of the per-tile counts of tiles
@@ -754,8 +757,8 @@ __global__ void __launch_bounds__(kDirectThreads)
 // particle is bounded by its own cell's occupancy.
 // ---------------------------------------------------------------------------------------------
 // The rank of particle me (old sorted slot p, bucket cell c) in the incremental bucketing (IncRank):
-// its place in the cell's old range when the cell's arrival word is of an earlier generation
-// (nothing entered or left it), else the members of the old range that stay in c and the cell's
+// its place in the cell's old range when the cell's mark or, where the mark matches, its arrival
+// word is of an earlier generation (nothing entered or left it), else the members of the old range that stay in c and the cell's
 // arrivals (id slots, then the overflow chain), counted where their id is smaller. The range is
 // clamped to the particles, so stale offsets cannot send the loop past the arrays.
 __device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, int32_t c, int32_t me,
@@ -764,8 +767,9 @@ __device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, i
                                              const int32_t* __restrict__ id,
                                              const int32_t* __restrict__ cell_ids,
                                              uint32_t id_stride, const int2* __restrict__ ovf) {
-  const uint64_t w = inc.arr[c];
   const uint32_t lo = inc.off_old[c];
+  if (inc.mark[c] != (uint8_t)gen) return p - lo;
+  const uint64_t w = inc.arr[c];
   if ((uint32_t)(w >> 32) != gen) return p - lo;
   const uint32_t hi = min(inc.off_old[c + 1], n);
   uint32_t rank = 0;
@@ -1885,14 +1889,14 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode, uint64_t* arr) {
+                            const TileCounts& tiles, int mode, uint64_t* arr, uint8_t* mark) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
-  if (arr && (!blocks || tiles.tc)) return hipErrorInvalidValue;  // id slots, look-back scan only
+  if (arr && (!blocks || tiles.tc || !mark)) return hipErrorInvalidValue;  // id slots, look-back scan
   unsigned long long* const arr_w = reinterpret_cast<unsigned long long*>(arr);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w)
+             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w, mark)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1932,6 +1936,15 @@ __global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, int6
   if (i < n) p[i] = 0u;
 }
 
+__global__ void __launch_bounds__(64) k_add_u32(uint32_t* __restrict__ p, uint32_t v) {
+  if (threadIdx.x == 0) p[0] += v;
+}
+
+hipError_t launch_add_u32(hipStream_t st, uint32_t* p, uint32_t v) {
+  hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
@@ -1948,9 +1961,9 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
 }
 
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear) {
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n) {
   PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ntiles,
-             ovf_ctr, flags, clear ? 1 : 0);
+             ovf_ctr, flags, clear ? 1 : 0, n);
   return hipGetLastError();
 }
 
@@ -1975,7 +1988,7 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               uint32_t* gen, const IncRank& inc) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
-  if (inc.arr && (!blocks || !inc.off_old || !inc.cell_cur_o)) return hipErrorInvalidValue;
+  if (inc.arr && (!blocks || !inc.off_old || !inc.cell_cur_o || !inc.mark)) return hipErrorInvalidValue;
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
              cell_ids, id_stride, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode,\
@@ -2211,7 +2224,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
-                                          const TileCounts&, int, uint64_t*);                          \
+                                          const TileCounts&, int, uint64_t*, uint8_t*);                \
   template hipError_t launch_front<T>(hipStream_t, int, int, const BlockInfo*,                  \
                                       const SceneDev<T>*, const SceneDev<T>*, bool, Vec4<T>*,  \
                                       Vec4<T>*, const int32_t*, const TriEntry<T>*,            \

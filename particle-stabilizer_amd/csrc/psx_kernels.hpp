@@ -7,7 +7,9 @@
 namespace psx {
 
 enum : int { kModeIntegrate = 1, kModeContainer = 2, kModeCheck = 4 };
-enum : uint32_t { kFlagNonFinite = 1u, kFlagScanStall = 2u };
+// kFlagCountDrift: the cell counts a scan read did not add up to the particle count (a bucketing
+// state that no longer matches the particles; the offsets were clamped to it).
+enum : uint32_t { kFlagNonFinite = 1u, kFlagScanStall = 2u, kFlagCountDrift = 4u };
 
 // Events that time the next kernel launch from its dispatch timestamps (consumed by the launch).
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
@@ -84,7 +86,10 @@ struct TileCounts {
 // (generation << 32) | arrivals: a particle that leaves or enters a cell raises the word to the
 // generation of the coming bucketing (ovf_ctr[2] + 1; the scan advances ovf_ctr[2] to it), an
 // entering one then adds itself to the arrivals and takes the id slot that the sum returns. The rank
-// kernel finds a cell that nothing entered or left by its stale generation. The bucket cell of the
+// kernel finds a cell that nothing entered or left by its stale generation. Such particles also
+// write the generation's low byte into both cells' entries of mark, which the rank kernel reads
+// first (one byte per cell instead of eight): only a cell whose mark matches can have changed, and
+// a stale match (a generation 256k earlier) is settled by the arrival word. The bucket cell of the
 // particle at sorted slot p is its home cell (home[p] & kHomeCell) unless kHomeMoved is set, then
 // cell_cur[p].
 struct NextBucket {
@@ -98,6 +103,7 @@ struct NextBucket {
   int32_t blocks;
   TileCounts tiles;
   unsigned long long* arr;  // incremental bucketing: per-cell arrival words (nullptr: full)
+  uint8_t* mark;            // incremental bucketing: per-cell low byte of the last change's generation
   const uint32_t* home;     // incremental bucketing: home words of the pair kernel's state
   const int32_t* cell_cur;  // incremental bucketing: bucket cells where kHomeMoved is set
 };
@@ -120,7 +126,8 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode, uint64_t* arr = nullptr);
+                            const TileCounts& tiles, int mode, uint64_t* arr = nullptr,
+                            uint8_t* mark = nullptr);
 
 // The first kernel of a step with PSX_FLAG_SEMANTICS_CUDA: integration, the container (box walls or
 // the CUDA backend's mesh contact), the state written back to pos/vel, then the hash pass of the
@@ -136,6 +143,8 @@ hipError_t launch_front(hipStream_t st, int ntab, int block, const BlockInfo* bt
 
 // p[0, n) = 0 (a kernel, so that it may sit in a captured stream).
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, int64_t n);
+// p[0] += v on the device (one thread), in stream order.
+hipError_t launch_add_u32(hipStream_t st, uint32_t* p, uint32_t v);
 
 // list[offset[c] + slot] = id[p] with c = cell_of[p] and slot = --count[c]: unordered dense cell
 // lists; the counts end at zero.
@@ -145,10 +154,12 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
 
 // One-pass exclusive scan of count[0, ntiles * kScanTile) into offset[0 .. ntiles * kScanTile]
 // (the last entry is the total); clears the counts when `clear`, moves ovf_ctr[0] to ovf_ctr[1],
-// zeroes ovf_ctr[0] and advances the generation ovf_ctr[2] (ovf_ctr holds three words). tile_state (ntiles words) persists between launches and is zeroed once at
+// zeroes ovf_ctr[0] and advances the generation ovf_ctr[2] (ovf_ctr holds three words). Every
+// offset is clamped to n (the particle count the counts must add up to; kFlagCountDrift when they
+// do not), so that no index derived from the offsets can leave the particle arrays. tile_state (ntiles words) persists between launches and is zeroed once at
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
-                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear);
+                       int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n);
 
 // The same scan over tiles.ntiles tiles of kDirectTile cells, each tile's prefix summed from the
 // per-tile counts of the hash pass (TileCounts); no tile waits for another.
@@ -163,6 +174,7 @@ hipError_t launch_scan_direct(hipStream_t st, uint32_t* count, uint32_t* offset,
 // slot whose home cell differs from it (kHomeMoved).
 struct IncRank {
   const uint64_t* arr = nullptr;
+  const uint8_t* mark = nullptr;
   const uint32_t* off_old = nullptr;
   int32_t* cell_cur_o = nullptr;
 };

@@ -295,9 +295,10 @@ struct SimImpl final : SimBase {
   int32_t ntiles_direct = 0;    // tiles of the direct scan, 0: look-back scan
   int32_t ntiles_direct_grid = 0;  // tiles of the direct scan when the grid and the form allow it
   DevBuf count, offset, tile_state, flags;
-  // incremental bucketing (sort_variant 3): per-cell arrival words, the second offset array (the
+  // incremental bucketing (sort_variant 3): per-cell arrival words and change marks, the second
+  // offset array (the
   // offsets alternate with the position buffers) and the bucket cells of container-moved spheres
-  DevBuf arr, offset2, cell_cur;
+  DevBuf arr, mark, offset2, cell_cur;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
@@ -564,9 +565,11 @@ struct SimImpl final : SimBase {
   void inc_buffers() {
     if (!incremental() || arr.p) return;
     arr.alloc((size_t)cpad * 8);
+    mark.alloc((size_t)cpad);
     offset2.alloc(offset.bytes);
     cell_cur.alloc((size_t)n * 4);
     PSX_HIP(hipMemset(arr.p, 0, arr.bytes));
+    PSX_HIP(hipMemset(mark.p, 0, mark.bytes));
     PSX_HIP(hipMemset(offset2.p, 0, offset2.bytes));
     PSX_HIP(hipDeviceSynchronize());
   }
@@ -626,7 +629,8 @@ struct SimImpl final : SimBase {
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
-                                 tiles(), mode, inc ? arr.as<uint64_t>() : nullptr);
+                                 tiles(), mode, inc ? arr.as<uint64_t>() : nullptr,
+                                 inc ? mark.as<uint8_t>() : nullptr);
     });
   }
 
@@ -644,6 +648,7 @@ struct SimImpl final : SimBase {
     IncRank inc;
     if (off_old) {
       inc.arr = arr.as<uint64_t>();
+      inc.mark = mark.as<uint8_t>();
       inc.off_old = off_old;
       inc.cell_cur_o = cell_cur.as<int32_t>();
     }
@@ -652,7 +657,8 @@ struct SimImpl final : SimBase {
         return launch_scan_direct(stream, count.as<uint32_t>(), off, tiles(), ovf_ctr.as<uint32_t>(),
                                   blocks);
       return launch_scan(stream, count.as<uint32_t>(), off, tile_state.as<uint64_t>(), ntiles,
-                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks && !off_old);
+                         ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), blocks && !off_old,
+                         (uint32_t)n);
     });
     if (!blocks)
       launch(4, prof, [&] {
@@ -711,6 +717,7 @@ struct SimImpl final : SimBase {
       nb.tiles = tiles();
       if (inc) {
         nb.arr = arr.as<unsigned long long>();
+        nb.mark = mark.as<uint8_t>();
         nb.home = home.as<uint32_t>();
         nb.cell_cur = cell_cur.as<int32_t>();
       }
@@ -738,12 +745,16 @@ struct SimImpl final : SimBase {
   bool hashed = false;
 
   // The incremental bucketing starts with every sphere arriving in its cell and with empty ranges
-  // in the offsets of the state (no sphere stays): those offsets are zeroed.
+  // in the offsets of the state (no sphere stays): those offsets are zeroed. Its generation is
+  // advanced first: a hash pass dropped since the last scan (drop_hash) has left arrival words of
+  // the generation the next scan would complete, which this pass must not add to.
   void ensure_hashed(Profiler* prof) {
     if (hashed || cuda) return;  // the CUDA backend's step hashes after its container
     const bool inc = incremental();
-    if (inc)
+    if (inc) {
+      PSX_HIP(launch_add_u32(stream, ovf_ctr.as<uint32_t>() + 2, 1u));
       PSX_HIP(launch_zero_u32(stream, offs_at(cur_pos) - kOffsetLead, (int64_t)(offset.bytes / 4)));
+    }
     hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
          kModeIntegrate, prof, inc);
     hashed = true;
@@ -1549,6 +1560,8 @@ int psx_synchronize(psx_sim* sim) {
     const uint32_t f = sim->impl->read_flags();
     if (f & psx::kFlagScanStall)
       return set_error(PSX_ERR_HIP, "cell-offset scan: a tile prefix was not published in time");
+    if (f & psx::kFlagCountDrift)
+      return set_error(PSX_ERR_HIP, "cell counts of a bucketing did not add up to the particle count");
     if (f & psx::kFlagNonFinite)
       return set_error(PSX_ERR_NUMERIC, "non-finite particle position or velocity produced by a step");
     return PSX_OK;

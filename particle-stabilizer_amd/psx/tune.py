@@ -10,9 +10,9 @@ each kernel keeps the shape with the smallest median time over its rounds. Tuned
 - (block_pairs, pair_variant, pair_chunk): workgroup size and lanes per particle of the pair
   kernel, or its LDS-staged form (pair_variant 32, workgroups of 64 to 256), and the one-lane
   kernel's candidates per round (4 or 2);
-- sort_variant: per-cell id slots or dense lists for the cell lists (half of the rounds each; the
-  form whose kernels take less time per step wins, the pair kernel included, since it does the
-  next step's hash pass).
+- sort_variant: per-cell id slots, dense lists or incremental id slots for the cell lists (a third
+  of the rounds each; the form whose kernels take less time per step wins, the pair kernel
+  included, since it does the next step's hash pass).
 
 The cell size stays what the reference's grid rule fixes (it changes the neighbour search, not
 only the speed), and every launch shape gives bit-identical results, so tuning never changes a
@@ -64,13 +64,14 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
                     "cached": True}
 
     p0, v0 = sim.state()
-    n_half = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
+    forms = (1, 2, 3)  # id slots, dense lists, incremental id slots
+    n_form = max(len(BLOCKS), len(PAIR_SHAPES)) * rounds_per_shape
     samples: dict[tuple, list] = {}
     launches: dict[tuple, list] = {}  # (kernel, sort variant) -> launches per step
     # the binary64 rank/gather kernel takes workgroups of at most 512
     sort_blocks = BLOCKS if np.dtype(sim.dtype) == np.float32 else tuple(b for b in BLOCKS if b <= 512)
-    for k in range(2 * n_half):  # first half per-cell id blocks, second half dense lists
-        sv = 1 if k < n_half else 2
+    for k in range(len(forms) * n_form):  # the rounds of each cell-list form in turn
+        sv = forms[k // n_form]
         bi = BLOCKS[k % len(BLOCKS)]
         bs = sort_blocks[(k // len(BLOCKS)) % len(sort_blocks)]
         bp, g, w = PAIR_SHAPES[k % len(PAIR_SHAPES)]
@@ -102,7 +103,7 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
 
     # per cell-list form: the fastest shape of each kernel, weighted by its launches per step
     plans = {}
-    for sv in (1, 2):
+    for sv in forms:
         bi_ms = {bi: med("k_integrate", bi, sv) for bi in BLOCKS}
         bi_ms = {b: t for b, t in bi_ms.items() if t is not None}
         bs_ms = {}
@@ -131,6 +132,7 @@ def autotune(sim, steps: int = 8, rounds_per_shape: int = 2, use_graph: bool = T
     sim.set_state(p0, v0)
     kernel_ms = {"step_id_blocks": plans.get(1, (None,))[0],
                  "step_dense_lists": plans.get(2, (None,))[0],
+                 "step_incremental": plans.get(3, (None,))[0],
                  "k_pairs": pair_best}
     if path is not None:
         data = _load(path)

@@ -14,15 +14,15 @@ def test_pair_kernel_bytes_of_the_262k_cylinder():
     # 262,144 spheres, 48 x 48 x 179 cells, binary32, id slots: 11 s N for the pair kick, 8 C for
     # the offset reads, 12 N for the hash pass's ids and cells and 8 N for its count updates (one
     # count read and written per sphere, not every cell of the grid)
-    b = bench.algorithmic_bytes(262144, 48 * 48 * 179, 4, True)
+    b = bench.algorithmic_bytes(262144, 48 * 48 * 179, 4, 1)
     assert b["k_pairs"] == 44 * 262144 + 8 * 412416 + 20 * 262144 == 20076544
     assert b["k_scan"] == 12 * 412416 and b["k_scatter_ids"] == 0
-    dense = bench.algorithmic_bytes(262144, 412416, 4, False)
+    dense = bench.algorithmic_bytes(262144, 412416, 4, 2)
     assert dense["k_scan"] == 8 * 412416
     assert dense["k_scatter_ids"] == 16 * 262144 + 8 * 412416
     assert dense["k_pairs"] == b["k_pairs"] - 4 * 262144  # no id filed in the dense form
     # binary64 moves twice the scalar bytes of every per-sphere array and the same index bytes
-    b64 = bench.algorithmic_bytes(262144, 412416, 8, True)
+    b64 = bench.algorithmic_bytes(262144, 412416, 8, 1)
     assert b64["k_pairs"] - b["k_pairs"] == 11 * 4 * 262144
     assert b64["k_rank_gather"] - b["k_rank_gather"] == 16 * 4 * 262144
 
@@ -31,7 +31,14 @@ def test_pair_kernel_bytes_of_the_hopper():
     # 4,194,304 spheres on the 179 x 179 x 358 grid: the per-launch figure behind the headline
     # roofline (64 N + 8 C)
     n, c = 4194304, 179 * 179 * 358
-    assert bench.algorithmic_bytes(n, c, 4, True)["k_pairs"] == 64 * n + 8 * c == 360200880
+    assert bench.algorithmic_bytes(n, c, 4, 1)["k_pairs"] == 64 * n + 8 * c == 360200880
+    assert bench.algorithmic_bytes(n, c, 4, 0) == bench.algorithmic_bytes(n, c, 4, 1)
+    # the incremental form: the hash pass writes the cells only (4 N), the scan reads the counts
+    # and writes the offsets (8 C), the rank kernel reads a mark and an old offset per sphere
+    inc = bench.algorithmic_bytes(n, c, 4, 3)
+    assert inc["k_pairs"] == 48 * n + 8 * c == 293092016
+    assert inc["k_scan"] == 8 * c
+    assert inc["k_rank_gather"] == bench.algorithmic_bytes(n, c, 4, 1)["k_rank_gather"] + 5 * n
 
 
 def test_cpu_threads_follow_the_cpu_share(monkeypatch):

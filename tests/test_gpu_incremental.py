@@ -8,7 +8,8 @@ oracle at that size):
 - a fast gas (most spheres change cell within a few steps, some by more than one cell per step);
 - the mesh scene, whose container pushes move spheres out of their bucket cells;
 - cells of 300 spheres and more than the eight id slots (arrivals on the overflow chains);
-- runs interleaved with statistics passes, state uploads and retunes between the forms;
+- runs interleaved with statistics passes, state uploads and retunes between the forms, retunes
+  that drop a pending hash pass with no scan after it, and the autotuner's sweep;
 - the full-size hopper, the contact-rich hopper pressed into its wall, the 256-scene batch and a
   crowded 1M-sphere box;
 - the differentiable window's gradients (the window's recompute uses the full form)."""
@@ -157,6 +158,42 @@ def test_runs_with_retunes_statistics_and_state_uploads(psx_mod, oracle, dtype):
         ref_p, ref_v = oracle_run(oracle, sc, sum(k for k, _ in RUNS))
         assert_bitwise(as_dtype(p, dtype), ref_p, "pos vs oracle")
         assert_bitwise(as_dtype(v, dtype), ref_v, "vel vs oracle")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_retunes_between_runs_without_a_scan(psx_mod, oracle, dtype):
+    """Retunes between short runs of the fast gas, each dropping the last step's pending hash pass
+    (many movers) with no scan after it, as the autotuner's rounds do: every first pass after a
+    drop takes a generation of its own, so the arrivals of the dropped pass are not counted."""
+    sc = fast_gas(dtype)
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        for r in range(8):
+            kw = dict(SHAPES[r % len(SHAPES)])
+            if dtype == "f64":
+                kw["block_sort"] = min(kw.get("block_sort", 256), 512)
+            s.set_tuning(use_graph=r % 2, sort_variant=3, **kw)
+            s.step(5)
+        s.synchronize()
+        p, v = s.state()
+    ref_p, ref_v = oracle_run(oracle, sc, 40)
+    assert_bitwise(as_dtype(p, dtype), ref_p, "pos")
+    assert_bitwise(as_dtype(v, dtype), ref_v, "vel")
+
+
+def test_autotune_rounds_then_run_bitexact(psx_mod, oracle):
+    """The autotuner's sweep (every launch shape of the three cell-list forms, a retune per round)
+    on the dense box, then a run from the restored state with the chosen shape."""
+    from psx.tune import autotune
+    sc = dense_box(dtype="f32")
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as s:
+        out = autotune(s, steps=2)
+        assert out["kernel_ms"]["step_incremental"] is not None
+        s.step(30)
+        s.synchronize()
+        p, v = s.state()
+    ref_p, ref_v = oracle_run(oracle, sc, 30)
+    assert_bitwise(as_dtype(p, "f32"), ref_p, "pos")
+    assert_bitwise(as_dtype(v, "f32"), ref_v, "vel")
 
 
 def test_batch_equals_full_form(psx_mod, oracle):

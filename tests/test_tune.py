@@ -59,7 +59,7 @@ def test_picks_fastest_shapes_and_restores_state():
     np.testing.assert_array_equal(sim.restored[0], sim.pos)
     np.testing.assert_array_equal(sim.restored[1], sim.vel)
     # every pair shape and every block size was timed
-    assert sim.profiled == len(tune.PAIR_SHAPES) * 2 * 2
+    assert sim.profiled == len(tune.PAIR_SHAPES) * 2 * 3
 
 
 def test_pair_kernel_cost_taken_per_cell_list_form():
@@ -90,6 +90,31 @@ def test_dense_lists_win_on_a_cheaper_pair_kernel():
             return out
 
     assert tune.autotune(DenseSim(), steps=4)["tuning"]["sort_variant"] == 2
+
+
+def test_incremental_form_wins_on_a_cheaper_pair_kernel_and_scan():
+    """The incremental id slots (sort_variant 3): a pair kernel without most of its hash pass's
+    atomics and a scan that leaves the counts alone, against a slower rank kernel."""
+    class IncSim(FakeSim):
+        def profile(self, steps):
+            out = super().profile(steps)
+            if self.tuning["sort_variant"] == 3:
+                out["k_pairs"]["ms"] -= 0.004
+                out["k_rank_gather"]["ms"] += 0.002
+            return out
+
+    out = tune.autotune(IncSim(), steps=4)
+    assert out["tuning"]["sort_variant"] == 3
+    assert out["kernel_ms"]["step_incremental"] < out["kernel_ms"]["step_id_blocks"]
+
+    class SlowRankSim(IncSim):
+        def profile(self, steps):
+            out = super().profile(steps)
+            if self.tuning["sort_variant"] == 3:
+                out["k_rank_gather"]["ms"] += 0.003  # more than the pair kernel saves
+            return out
+
+    assert tune.autotune(SlowRankSim(), steps=4)["tuning"]["sort_variant"] == 1
 
 
 def test_two_wide_chunks_chosen_when_faster():
