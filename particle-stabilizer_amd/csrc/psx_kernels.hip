@@ -322,8 +322,8 @@ __device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextB
     const uint32_t hw = nb.home[p];
     const int32_t c_old = (hw & kHomeMoved) ? nb.cell_cur[p] : (int32_t)(hw & kHomeCell);
     if (c != c_old)
-      bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.mark, nb.cell_ids,
-                  nb.id_stride, nb.ovf, nb.ovf_ctr);
+      bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.mark, nb.tt,
+                  nb.cell_ids, nb.id_stride, nb.ovf, nb.ovf_ctr);
     return c;
   }
   return bucket_entry(s, x, nb.id[p], p, nb.blocks != 0, nb.cell_of, nb.count, nb.cell_ids,
@@ -388,7 +388,8 @@ __global__ void __launch_bounds__(1024)
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
                 uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode,
-                unsigned long long* __restrict__ arr, uint8_t* __restrict__ mark) {
+                unsigned long long* __restrict__ arr, uint8_t* __restrict__ mark,
+                const TileTotals tt) {
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -403,11 +404,13 @@ __global__ void __launch_bounds__(1024)
       if (kBlocks && arr) {  // first pass of the incremental bucketing: every particle arrives
         c = grid_cell(s, x);
         cell_of[p] = c;
-        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, mark, cell_ids, id_stride, ovf, ovf_ctr);
+        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, mark, tt, cell_ids, id_stride, ovf,
+                    ovf_ctr);
       } else {
         c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
       }
     }
+    if (kBlocks && arr) tile_total_wave(tt, c);
     if (tiles.tc) {
       wt.begin(c);
       wt.add(tiles, c);
@@ -677,6 +680,82 @@ __global__ void __launch_bounds__(kScanThreads)
     // total: the offset one past the last cell
     offset[(size_t)ntiles * kScanTile] = min(round_base, limit);
     if (round_base != limit) atomicOr(flags, kFlagCountDrift);
+  }
+}
+
+// The incremental bucketing's scan (TileTotals): tile b's prefix is the sum of the kept totals of
+// tiles [0, b) (the generation's entries, read in parallel by the workgroup's threads), so no tile
+// waits for another. Each tile writes its own total, counted from its cells, into the other
+// generation's entry for the next bucketing and raises kFlagCountDrift where the kept total
+// differs from it. Offsets are clamped to `limit` as in k_scan; the counts are left in place.
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_tiles(const uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
+                 const TileTotals tt, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
+                 uint32_t* __restrict__ flags, uint32_t limit) {
+  constexpr int kScanRounds = kScanPerThread / 4;
+  constexpr int kWaves = kScanThreads / 64;
+  __shared__ uint32_t wsum[kScanRounds][kWaves];
+  __shared__ uint32_t wpre[kWaves];
+  const int32_t tile = (int32_t)blockIdx.x;
+  const uint32_t par = *tt.gen & 1u;
+  const uint32_t* __restrict__ tin = tt.tot + (size_t)par * (size_t)ntiles;
+  uint32_t* __restrict__ tout = tt.tot + (size_t)(par ^ 1u) * (size_t)ntiles;
+  if (tile == 0 && threadIdx.x == 0) {
+    ovf_ctr[1] = ovf_ctr[0];
+    ovf_ctr[0] = 0u;
+    ovf_ctr[2] += 1u;
+  }
+  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kScanTile);
+  uint4 u[kScanRounds];
+  const int w = threadIdx.x >> 6;
+  const int lane = lane_id();
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) u[q] = src[q * kScanThreads + threadIdx.x];
+  uint32_t pre = 0;
+  for (int32_t k = (int32_t)threadIdx.x; k < tile; k += kScanThreads) pre += tin[k];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) pre += __shfl_xor(pre, d, 64);
+  if (lane == 0) wpre[w] = pre;
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) {
+    const uint32_t wi = wave_inclusive_scan((u[q].x + u[q].y) + (u[q].z + u[q].w));
+    if (lane == 63) wsum[q][w] = wi;
+  }
+  __syncthreads();
+  uint32_t prefix = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) prefix += wpre[k];
+  uint4* __restrict__ dst = reinterpret_cast<uint4*>(offset + (size_t)tile * kScanTile);
+  uint32_t round_base = prefix;
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) {
+    const uint32_t tot = (u[q].x + u[q].y) + (u[q].z + u[q].w);
+    uint32_t run = round_base + wave_inclusive_scan(tot) - tot;
+    uint32_t rtot = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      run += (k < w) ? wsum[q][k] : 0u;
+      rtot += wsum[q][k];
+    }
+    uint4 o;
+    o.x = min(run, limit);
+    run += u[q].x;
+    o.y = min(run, limit);
+    run += u[q].y;
+    o.z = min(run, limit);
+    run += u[q].z;
+    o.w = min(run, limit);
+    dst[q * kScanThreads + threadIdx.x] = o;
+    round_base += rtot;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t agg = round_base - prefix;
+    tout[tile] = agg;
+    if (agg != tin[tile]) atomicOr(flags, kFlagCountDrift);
+    if (tile == ntiles - 1) {
+      offset[(size_t)ntiles * kScanTile] = min(round_base, limit);
+      if (round_base != limit) atomicOr(flags, kFlagCountDrift);
+    }
   }
 }
 
@@ -1889,14 +1968,15 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode, uint64_t* arr, uint8_t* mark) {
+                            const TileCounts& tiles, int mode, uint64_t* arr, uint8_t* mark,
+                            const TileTotals& tt) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
-  if (arr && (!blocks || tiles.tc || !mark)) return hipErrorInvalidValue;  // id slots, look-back scan
+  if (arr && (!blocks || tiles.tc || !mark || !tt.tot)) return hipErrorInvalidValue;  // id slots, no direct scan
   unsigned long long* const arr_w = reinterpret_cast<unsigned long long*>(arr);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w, mark)
+             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w, mark, tt)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -1964,6 +2044,14 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n) {
   PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ntiles,
              ovf_ctr, flags, clear ? 1 : 0, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_tiles(hipStream_t st, const uint32_t* count, uint32_t* offset,
+                             const TileTotals& tt, int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags,
+                             uint32_t n) {
+  PSX_LAUNCH(k_scan_tiles, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tt, ntiles,
+             ovf_ctr, flags, n);
   return hipGetLastError();
 }
 
@@ -2224,7 +2312,8 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
-                                          const TileCounts&, int, uint64_t*, uint8_t*);                \
+                                          const TileCounts&, int, uint64_t*, uint8_t*,                 \
+                                          const TileTotals&);                                          \
   template hipError_t launch_front<T>(hipStream_t, int, int, const BlockInfo*,                  \
                                       const SceneDev<T>*, const SceneDev<T>*, bool, Vec4<T>*,  \
                                       Vec4<T>*, const int32_t*, const TriEntry<T>*,            \

@@ -80,6 +80,17 @@ struct TileCounts {
   int32_t ntiles;
 };
 
+// Kept per-tile totals of the cell counts (incremental bucketing): two generations of ntiles words
+// (tile t of kScanTile cells; generation parity *gen & 1, gen advancing once per bucketing in the
+// rank/gather kernel). A hash pass adds to the current generation's entries (the first pass every
+// particle, then only the particles that change tile); the scan sums them for its prefixes and
+// writes the next generation's from the counts.
+struct TileTotals {
+  uint32_t* tot = nullptr;
+  const uint32_t* gen = nullptr;
+  int32_t ntiles = 0;
+};
+
 // Incremental bucketing (arr != nullptr; look-back-scan grids, id slots): the counts persist from
 // one step to the next and only a particle whose next cell differs from its bucket cell touches
 // them (one less in the old cell, one more in the new). arr holds one word per cell,
@@ -106,6 +117,7 @@ struct NextBucket {
   uint8_t* mark;            // incremental bucketing: per-cell low byte of the last change's generation
   const uint32_t* home;     // incremental bucketing: home words of the pair kernel's state
   const int32_t* cell_cur;  // incremental bucketing: bucket cells where kHomeMoved is set
+  TileTotals tt;            // incremental bucketing: kept tile totals
 };
 
 // Per-particle kernels: with `one` (a single scene, its host copy) the workgroups need no table
@@ -127,7 +139,7 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
                             const TileCounts& tiles, int mode, uint64_t* arr = nullptr,
-                            uint8_t* mark = nullptr);
+                            uint8_t* mark = nullptr, const TileTotals& tt = TileTotals{});
 
 // The first kernel of a step with PSX_FLAG_SEMANTICS_CUDA: integration, the container (box walls or
 // the CUDA backend's mesh contact), the state written back to pos/vel, then the hash pass of the
@@ -160,6 +172,13 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n);
+
+// The incremental bucketing's scan over ntiles tiles of kScanTile cells (counts left in place):
+// each tile's prefix is summed from the kept tile totals of the current generation (TileTotals), and
+// each tile writes the total of its counts for the next; kFlagCountDrift where the two differ.
+hipError_t launch_scan_tiles(hipStream_t st, const uint32_t* count, uint32_t* offset,
+                             const TileTotals& tt, int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags,
+                             uint32_t n);
 
 // The same scan over tiles.ntiles tiles of kDirectTile cells, each tile's prefix summed from the
 // per-tile counts of the hash pass (TileCounts); no tile waits for another.

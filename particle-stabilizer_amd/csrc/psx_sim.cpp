@@ -296,9 +296,9 @@ struct SimImpl final : SimBase {
   int32_t ntiles_direct_grid = 0;  // tiles of the direct scan when the grid and the form allow it
   DevBuf count, offset, tile_state, flags;
   // incremental bucketing (sort_variant 3): per-cell arrival words and change marks, the second
-  // offset array (the
-  // offsets alternate with the position buffers) and the bucket cells of container-moved spheres
-  DevBuf arr, mark, offset2, cell_cur;
+  // offset array (the offsets alternate with the position buffers), the bucket cells of
+  // container-moved spheres and the kept per-tile totals of the counts (two generations)
+  DevBuf arr, mark, offset2, cell_cur, tile_tot;
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
@@ -568,10 +568,19 @@ struct SimImpl final : SimBase {
     mark.alloc((size_t)cpad);
     offset2.alloc(offset.bytes);
     cell_cur.alloc((size_t)n * 4);
+    tile_tot.alloc((size_t)2 * ntiles * 4);
     PSX_HIP(hipMemset(arr.p, 0, arr.bytes));
     PSX_HIP(hipMemset(mark.p, 0, mark.bytes));
     PSX_HIP(hipMemset(offset2.p, 0, offset2.bytes));
     PSX_HIP(hipDeviceSynchronize());
+  }
+
+  TileTotals tile_totals() const {
+    TileTotals t;
+    t.tot = tile_tot.as<uint32_t>();
+    t.gen = scan_gen.as<uint32_t>();
+    t.ntiles = ntiles;
+    return t;
   }
 
   // Offsets of the incremental bucketing's state in position buffer k (k = 0: the offsets every
@@ -630,7 +639,8 @@ struct SimImpl final : SimBase {
                                  cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
                                  tiles(), mode, inc ? arr.as<uint64_t>() : nullptr,
-                                 inc ? mark.as<uint8_t>() : nullptr);
+                                 inc ? mark.as<uint8_t>() : nullptr,
+                                 inc ? tile_totals() : TileTotals{});
     });
   }
 
@@ -653,6 +663,9 @@ struct SimImpl final : SimBase {
       inc.cell_cur_o = cell_cur.as<int32_t>();
     }
     launch(1, prof, [&] {
+      if (off_old)
+        return launch_scan_tiles(stream, count.as<uint32_t>(), off, tile_totals(), ntiles,
+                                 ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), (uint32_t)n);
       if (ntiles_direct)
         return launch_scan_direct(stream, count.as<uint32_t>(), off, tiles(), ovf_ctr.as<uint32_t>(),
                                   blocks);
@@ -720,6 +733,7 @@ struct SimImpl final : SimBase {
         nb.mark = mark.as<uint8_t>();
         nb.home = home.as<uint32_t>();
         nb.cell_cur = cell_cur.as<int32_t>();
+        nb.tt = tile_totals();
       }
     }
     return nb;
@@ -747,13 +761,15 @@ struct SimImpl final : SimBase {
   // The incremental bucketing starts with every sphere arriving in its cell and with empty ranges
   // in the offsets of the state (no sphere stays): those offsets are zeroed. Its generation is
   // advanced first: a hash pass dropped since the last scan (drop_hash) has left arrival words of
-  // the generation the next scan would complete, which this pass must not add to.
+  // the generation the next scan would complete, which this pass must not add to. The kept tile
+  // totals start from zero (the pass adds every sphere to its tile's).
   void ensure_hashed(Profiler* prof) {
     if (hashed || cuda) return;  // the CUDA backend's step hashes after its container
     const bool inc = incremental();
     if (inc) {
       PSX_HIP(launch_add_u32(stream, ovf_ctr.as<uint32_t>() + 2, 1u));
       PSX_HIP(launch_zero_u32(stream, offs_at(cur_pos) - kOffsetLead, (int64_t)(offset.bytes / 4)));
+      PSX_HIP(launch_zero_u32(stream, tile_tot.as<uint32_t>(), 2 * (int64_t)ntiles));
     }
     hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
          kModeIntegrate, prof, inc);
