@@ -71,7 +71,7 @@ __device__ __forceinline__ int32_t bucket_entry(const SceneDev<T>& s, const Vec4
 // every addition of a generation lands on a word that already carries it. Both cells' marks take
 // the generation's low byte (plain stores: every writer of a generation stores the same value). A
 // particle that changes scan tile also moves one from its old tile's kept total to the new one's
-// (TileTotals); the first pass adds its tile totals per wave instead (tile_total_wave).
+// (TileTotals); after the first pass (c_old < 0) the totals are summed from the counts instead.
 __device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old, uint32_t tag,
                                             uint32_t* __restrict__ count,
                                             unsigned long long* __restrict__ arr,
@@ -96,24 +96,6 @@ __device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old
   mark[c] = (uint8_t)tag;
   const uint32_t slot = (uint32_t)atomicAdd(arr + c, 1ull);
   file_id(me, c, slot, cell_ids, stride, ovf, ovf_ctr);
-}
-
-// The first pass of the incremental bucketing: adds the calling lanes' particles (cells c; c < 0:
-// none) to the kept totals of their scan tiles, one atomic for the wave where its particles share a
-// tile.
-__device__ __forceinline__ void tile_total_wave(const TileTotals& tt, int32_t c) {
-  const bool has = c >= 0;
-  const uint64_t act = __ballot(has);
-  if (!has) return;
-  uint32_t* __restrict__ tot = tt.tot + (size_t)(*tt.gen & 1u) * (size_t)tt.ntiles;
-  const int leader = __ffsll((unsigned long long)act) - 1;
-  const uint32_t t = (uint32_t)c / (uint32_t)kScanTile;
-  const uint32_t t0 = __shfl(t, leader, 64);
-  if (__ballot(t == t0) == act) {
-    if (lane_id() == leader) atomicAdd(tot + t0, (uint32_t)__popcll(act));
-  } else {
-    atomicAdd(tot + t, 1u);
-  }
 }
 
 // Adds the calling lanes' particles (cells c) to their tiles' counts for the direct scan

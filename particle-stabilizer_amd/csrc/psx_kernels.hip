@@ -388,8 +388,7 @@ __global__ void __launch_bounds__(1024)
                 int32_t* __restrict__ cell_of, uint32_t* __restrict__ count,
                 int32_t* __restrict__ cell_ids, uint32_t id_stride, int2* __restrict__ ovf,
                 uint32_t* __restrict__ ovf_ctr, const TileCounts tiles, int mode,
-                unsigned long long* __restrict__ arr, uint8_t* __restrict__ mark,
-                const TileTotals tt) {
+                unsigned long long* __restrict__ arr, uint8_t* __restrict__ mark) {
   __shared__ WgTiles wt;
   with_slice<T, kTable>(loc, [&](const SceneDev<T>& s, int32_t p0, int32_t p1) {
     const int32_t p = p0 + (int32_t)threadIdx.x;
@@ -404,13 +403,12 @@ __global__ void __launch_bounds__(1024)
       if (kBlocks && arr) {  // first pass of the incremental bucketing: every particle arrives
         c = grid_cell(s, x);
         cell_of[p] = c;
-        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, mark, tt, cell_ids, id_stride, ovf,
-                    ovf_ctr);
+        bucket_move(me, c, -1, ovf_ctr[2] + 1u, count, arr, mark, TileTotals{}, cell_ids, id_stride,
+                    ovf, ovf_ctr);
       } else {
         c = bucket_entry(s, x, me, p, kBlocks, cell_of, count, cell_ids, id_stride, ovf, ovf_ctr);
       }
     }
-    if (kBlocks && arr) tile_total_wave(tt, c);
     if (tiles.tc) {
       wt.begin(c);
       wt.add(tiles, c);
@@ -680,6 +678,33 @@ __global__ void __launch_bounds__(kScanThreads)
     // total: the offset one past the last cell
     offset[(size_t)ntiles * kScanTile] = min(round_base, limit);
     if (round_base != limit) atomicOr(flags, kFlagCountDrift);
+  }
+}
+
+// The kept tile totals after the incremental bucketing's first pass (TileTotals): each tile's
+// counts summed into the current generation's entry, one workgroup per tile.
+__global__ void __launch_bounds__(kScanThreads)
+    k_tile_totals(const uint32_t* __restrict__ count, const TileTotals tt) {
+  constexpr int kScanRounds = kScanPerThread / 4;
+  constexpr int kWaves = kScanThreads / 64;
+  __shared__ uint32_t wsum[kWaves];
+  const int32_t tile = (int32_t)blockIdx.x;
+  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kScanTile);
+  uint4 u[kScanRounds];
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) u[q] = src[q * kScanThreads + threadIdx.x];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < kScanRounds; ++q) sum += (u[q].x + u[q].y) + (u[q].z + u[q].w);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) total += wsum[k];
+    tt.tot[(size_t)(*tt.gen & 1u) * (size_t)tt.ntiles + (size_t)tile] = total;
   }
 }
 
@@ -1968,15 +1993,14 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             const Vec4<T>* pos, const Vec4<T>* vel, const int32_t* id,
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
-                            const TileCounts& tiles, int mode, uint64_t* arr, uint8_t* mark,
-                            const TileTotals& tt) {
+                            const TileCounts& tiles, int mode, uint64_t* arr, uint8_t* mark) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
-  if (arr && (!blocks || tiles.tc || !mark || !tt.tot)) return hipErrorInvalidValue;  // id slots, no direct scan
+  if (arr && (!blocks || tiles.tc || !mark)) return hipErrorInvalidValue;  // id slots, no direct scan
   unsigned long long* const arr_w = reinterpret_cast<unsigned long long*>(arr);
 #define PSX_INTEGRATE(TABLE, BLOCKS)                                                             \
   PSX_LAUNCH((k_integrate<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, pos, vel, id, cell_of,     \
-             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w, mark, tt)
+             count, cell_ids, id_stride, ovf, ovf_ctr, tiles, mode, arr_w, mark)
   if (one) {
     if (blocks) PSX_INTEGRATE(false, true);
     else PSX_INTEGRATE(false, false);
@@ -2044,6 +2068,11 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n) {
   PSX_LAUNCH(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tile_state, ntiles,
              ovf_ctr, flags, clear ? 1 : 0, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_totals(hipStream_t st, const uint32_t* count, const TileTotals& tt) {
+  PSX_LAUNCH(k_tile_totals, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
   return hipGetLastError();
 }
 
@@ -2312,8 +2341,7 @@ hipError_t launch_sort_in(hipStream_t st, int32_t n, Vec4<T>* pos, Vec4<T>* vel,
                                           const SceneDev<T>*, const SceneDev<T>*, bool,        \
                                           const Vec4<T>*, const Vec4<T>*, const int32_t*,      \
                                           int32_t*, uint32_t*, int32_t*, uint32_t, int2*, uint32_t*,     \
-                                          const TileCounts&, int, uint64_t*, uint8_t*,                 \
-                                          const TileTotals&);                                          \
+                                          const TileCounts&, int, uint64_t*, uint8_t*);                \
   template hipError_t launch_front<T>(hipStream_t, int, int, const BlockInfo*,                  \
                                       const SceneDev<T>*, const SceneDev<T>*, bool, Vec4<T>*,  \
                                       Vec4<T>*, const int32_t*, const TriEntry<T>*,            \

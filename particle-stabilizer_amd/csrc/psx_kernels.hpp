@@ -82,9 +82,9 @@ struct TileCounts {
 
 // Kept per-tile totals of the cell counts (incremental bucketing): two generations of ntiles words
 // (tile t of kScanTile cells; generation parity *gen & 1, gen advancing once per bucketing in the
-// rank/gather kernel). A hash pass adds to the current generation's entries (the first pass every
-// particle, then only the particles that change tile); the scan sums them for its prefixes and
-// writes the next generation's from the counts.
+// rank/gather kernel). launch_tile_totals sums the counts of the first pass into the current
+// generation's entries; later hash passes move one between two entries for each particle that
+// changes tile; the scan sums them for its prefixes and writes the next generation's from the counts.
 struct TileTotals {
   uint32_t* tot = nullptr;
   const uint32_t* gen = nullptr;
@@ -139,7 +139,7 @@ hipError_t launch_integrate(hipStream_t st, int ntab, int block, const BlockInfo
                             int32_t* cell_of, uint32_t* count, int32_t* cell_ids,
                             uint32_t id_stride, int2* ovf, uint32_t* ovf_ctr,
                             const TileCounts& tiles, int mode, uint64_t* arr = nullptr,
-                            uint8_t* mark = nullptr, const TileTotals& tt = TileTotals{});
+                            uint8_t* mark = nullptr);
 
 // The first kernel of a step with PSX_FLAG_SEMANTICS_CUDA: integration, the container (box walls or
 // the CUDA backend's mesh contact), the state written back to pos/vel, then the hash pass of the
@@ -172,6 +172,10 @@ hipError_t launch_scatter_ids(hipStream_t st, int block, int32_t n, const int32_
 // allocation; launches on one handle must be serialized.
 hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64_t* tile_state,
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n);
+
+// The kept tile totals of the incremental bucketing's first pass: the counts of each of tt.ntiles
+// tiles of kScanTile cells summed into the current generation's entry (TileTotals).
+hipError_t launch_tile_totals(hipStream_t st, const uint32_t* count, const TileTotals& tt);
 
 // The incremental bucketing's scan over ntiles tiles of kScanTile cells (counts left in place):
 // each tile's prefix is summed from the kept tile totals of the current generation (TileTotals), and

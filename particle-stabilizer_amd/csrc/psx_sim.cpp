@@ -639,8 +639,7 @@ struct SimImpl final : SimBase {
                                  cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
                                  tiles(), mode, inc ? arr.as<uint64_t>() : nullptr,
-                                 inc ? mark.as<uint8_t>() : nullptr,
-                                 inc ? tile_totals() : TileTotals{});
+                                 inc ? mark.as<uint8_t>() : nullptr);
     });
   }
 
@@ -762,17 +761,18 @@ struct SimImpl final : SimBase {
   // in the offsets of the state (no sphere stays): those offsets are zeroed. Its generation is
   // advanced first: a hash pass dropped since the last scan (drop_hash) has left arrival words of
   // the generation the next scan would complete, which this pass must not add to. The kept tile
-  // totals start from zero (the pass adds every sphere to its tile's).
+  // totals are then summed from the pass's counts (a reduction, where per-sphere atomics on a few
+  // hundred totals would contend on an unsorted state).
   void ensure_hashed(Profiler* prof) {
     if (hashed || cuda) return;  // the CUDA backend's step hashes after its container
     const bool inc = incremental();
     if (inc) {
       PSX_HIP(launch_add_u32(stream, ovf_ctr.as<uint32_t>() + 2, 1u));
       PSX_HIP(launch_zero_u32(stream, offs_at(cur_pos) - kOffsetLead, (int64_t)(offset.bytes / 4)));
-      PSX_HIP(launch_zero_u32(stream, tile_tot.as<uint32_t>(), 2 * (int64_t)ntiles));
     }
     hash(pos[cur_pos].as<Vec4<T>>(), vel[cur_vel].as<Vec4<T>>(), id[cur_pos].as<int32_t>(),
          kModeIntegrate, prof, inc);
+    if (inc) PSX_HIP(launch_tile_totals(stream, count.as<uint32_t>(), tile_totals()));
     hashed = true;
   }
 
