@@ -84,7 +84,7 @@ __device__ __forceinline__ void bucket_move(int32_t me, int32_t c, int32_t c_old
     atomicSub(count + c_old, 1u);
     atomicMax(arr + c_old, t);
     mark[c_old] = (uint8_t)tag;
-    const uint32_t to = (uint32_t)c_old / (uint32_t)kScanTile, tn = (uint32_t)c / (uint32_t)kScanTile;
+    const uint32_t to = (uint32_t)c_old >> tt.shift, tn = (uint32_t)c >> tt.shift;
     if (to != tn) {
       uint32_t* __restrict__ tot = tt.tot + (size_t)(*tt.gen & 1u) * (size_t)tt.ntiles;
       atomicSub(tot + to, 1u);

@@ -682,14 +682,17 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 // The kept tile totals after the incremental bucketing's first pass (TileTotals): each tile's
-// counts summed into the current generation's entry, one workgroup per tile.
+// counts summed into the current generation's entry, one workgroup per tile of kRounds sub-tiles of
+// kScanThreads uint4 (kScanTile cells for 16 rounds, kDirectTile for one).
+template <int kRounds>
 __global__ void __launch_bounds__(kScanThreads)
     k_tile_totals(const uint32_t* __restrict__ count, const TileTotals tt) {
-  constexpr int kScanRounds = kScanPerThread / 4;
+  constexpr int kScanRounds = kRounds;
   constexpr int kWaves = kScanThreads / 64;
+  constexpr size_t kTile = (size_t)kScanThreads * 4 * kRounds;
   __shared__ uint32_t wsum[kWaves];
   const int32_t tile = (int32_t)blockIdx.x;
-  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kScanTile);
+  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kTile);
   uint4 u[kScanRounds];
 #pragma unroll
   for (int q = 0; q < kScanRounds; ++q) u[q] = src[q * kScanThreads + threadIdx.x];
@@ -713,12 +716,17 @@ __global__ void __launch_bounds__(kScanThreads)
 // waits for another. Each tile writes its own total, counted from its cells, into the other
 // generation's entry for the next bucketing and raises kFlagCountDrift where the kept total
 // differs from it. Offsets are clamped to `limit` as in k_scan; the counts are left in place.
+// Tiles of kRounds sub-tiles as in k_tile_totals: kScanTile cells on large grids, kDirectTile cells
+// on grids of at most kDirectTiles of them (more workgroups, each summing at most kDirectTiles
+// totals).
+template <int kRounds>
 __global__ void __launch_bounds__(kScanThreads)
     k_scan_tiles(const uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
                  const TileTotals tt, int32_t ntiles, uint32_t* __restrict__ ovf_ctr,
                  uint32_t* __restrict__ flags, uint32_t limit) {
-  constexpr int kScanRounds = kScanPerThread / 4;
+  constexpr int kScanRounds = kRounds;
   constexpr int kWaves = kScanThreads / 64;
+  constexpr size_t kTile = (size_t)kScanThreads * 4 * kRounds;
   __shared__ uint32_t wsum[kScanRounds][kWaves];
   __shared__ uint32_t wpre[kWaves];
   const int32_t tile = (int32_t)blockIdx.x;
@@ -730,7 +738,7 @@ __global__ void __launch_bounds__(kScanThreads)
     ovf_ctr[0] = 0u;
     ovf_ctr[2] += 1u;
   }
-  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kScanTile);
+  const uint4* __restrict__ src = reinterpret_cast<const uint4*>(count + (size_t)tile * kTile);
   uint4 u[kScanRounds];
   const int w = threadIdx.x >> 6;
   const int lane = lane_id();
@@ -750,7 +758,7 @@ __global__ void __launch_bounds__(kScanThreads)
   uint32_t prefix = 0;
 #pragma unroll
   for (int k = 0; k < kWaves; ++k) prefix += wpre[k];
-  uint4* __restrict__ dst = reinterpret_cast<uint4*>(offset + (size_t)tile * kScanTile);
+  uint4* __restrict__ dst = reinterpret_cast<uint4*>(offset + (size_t)tile * kTile);
   uint32_t round_base = prefix;
 #pragma unroll
   for (int q = 0; q < kScanRounds; ++q) {
@@ -778,7 +786,7 @@ __global__ void __launch_bounds__(kScanThreads)
     tout[tile] = agg;
     if (agg != tin[tile]) atomicOr(flags, kFlagCountDrift);
     if (tile == ntiles - 1) {
-      offset[(size_t)ntiles * kScanTile] = min(round_base, limit);
+      offset[(size_t)ntiles * kTile] = min(round_base, limit);
       if (round_base != limit) atomicOr(flags, kFlagCountDrift);
     }
   }
@@ -2072,15 +2080,26 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 }
 
 hipError_t launch_tile_totals(hipStream_t st, const uint32_t* count, const TileTotals& tt) {
-  PSX_LAUNCH(k_tile_totals, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
+  if (tt.shift == kDirectShift)
+    PSX_LAUNCH(k_tile_totals<1>, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
+  else if (tt.shift == kScanShift)
+    PSX_LAUNCH(k_tile_totals<kScanPerThread / 4>, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launch_scan_tiles(hipStream_t st, const uint32_t* count, uint32_t* offset,
                              const TileTotals& tt, int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags,
                              uint32_t n) {
-  PSX_LAUNCH(k_scan_tiles, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tt, ntiles,
-             ovf_ctr, flags, n);
+  if (tt.shift == kDirectShift)
+    PSX_LAUNCH(k_scan_tiles<1>, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tt, ntiles,
+               ovf_ctr, flags, n);
+  else if (tt.shift == kScanShift)
+    PSX_LAUNCH(k_scan_tiles<kScanPerThread / 4>, dim3(ntiles), dim3(kScanThreads), 0, st, count,
+               offset, tt, ntiles, ovf_ctr, flags, n);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -20,6 +20,8 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 64;
 constexpr int kScanTile = kScanThreads * kScanPerThread;  // cells per scan tile
+constexpr int kScanShift = 14;                            // log2(kScanTile)
+static_assert(kScanTile == 1 << kScanShift, "kScanShift must match kScanTile");
 // Cells per id-slot plane are rounded to this rather than to the scan tile, so the id slots take
 // no more memory than the cells need.
 constexpr int64_t kIdPlane = 4096;
@@ -81,7 +83,7 @@ struct TileCounts {
 };
 
 // Kept per-tile totals of the cell counts (incremental bucketing): two generations of ntiles words
-// (tile t of kScanTile cells; generation parity *gen & 1, gen advancing once per bucketing in the
+// (tile t of 2^shift cells; generation parity *gen & 1, gen advancing once per bucketing in the
 // rank/gather kernel). launch_tile_totals sums the counts of the first pass into the current
 // generation's entries; later hash passes move one between two entries for each particle that
 // changes tile; the scan sums them for its prefixes and writes the next generation's from the counts.
@@ -89,6 +91,7 @@ struct TileTotals {
   uint32_t* tot = nullptr;
   const uint32_t* gen = nullptr;
   int32_t ntiles = 0;
+  int32_t shift = 0;  // log2 of the cells per tile: kScanShift, or kDirectShift on small grids
 };
 
 // Incremental bucketing (arr != nullptr; look-back-scan grids, id slots): the counts persist from
@@ -174,10 +177,10 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
                        int32_t ntiles, uint32_t* ovf_ctr, uint32_t* flags, bool clear, uint32_t n);
 
 // The kept tile totals of the incremental bucketing's first pass: the counts of each of tt.ntiles
-// tiles of kScanTile cells summed into the current generation's entry (TileTotals).
+// tiles of 2^tt.shift cells summed into the current generation's entry (TileTotals).
 hipError_t launch_tile_totals(hipStream_t st, const uint32_t* count, const TileTotals& tt);
 
-// The incremental bucketing's scan over ntiles tiles of kScanTile cells (counts left in place):
+// The incremental bucketing's scan over ntiles tiles of 2^tt.shift cells (counts left in place):
 // each tile's prefix is summed from the kept tile totals of the current generation (TileTotals), and
 // each tile writes the total of its counts for the next; kFlagCountDrift where the two differ.
 hipError_t launch_scan_tiles(hipStream_t st, const uint32_t* count, uint32_t* offset,

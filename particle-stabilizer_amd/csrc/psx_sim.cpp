@@ -568,18 +568,24 @@ struct SimImpl final : SimBase {
     mark.alloc((size_t)cpad);
     offset2.alloc(offset.bytes);
     cell_cur.alloc((size_t)n * 4);
-    tile_tot.alloc((size_t)2 * ntiles * 4);
+    tile_tot.alloc((size_t)2 * inc_tiles() * 4);
     PSX_HIP(hipMemset(arr.p, 0, arr.bytes));
     PSX_HIP(hipMemset(mark.p, 0, mark.bytes));
     PSX_HIP(hipMemset(offset2.p, 0, offset2.bytes));
     PSX_HIP(hipDeviceSynchronize());
   }
 
+  // Tiles of the incremental bucketing's scan: kDirectTile cells on grids the direct scan covers
+  // (more workgroups than 16,384-cell tiles would give), kScanTile cells on larger ones.
+  int32_t inc_shift() const { return ntiles_direct_grid ? kDirectShift : kScanShift; }
+  int32_t inc_tiles() const { return (int32_t)(cpad >> inc_shift()); }
+
   TileTotals tile_totals() const {
     TileTotals t;
     t.tot = tile_tot.as<uint32_t>();
     t.gen = scan_gen.as<uint32_t>();
-    t.ntiles = ntiles;
+    t.ntiles = inc_tiles();
+    t.shift = inc_shift();
     return t;
   }
 
@@ -663,7 +669,7 @@ struct SimImpl final : SimBase {
     }
     launch(1, prof, [&] {
       if (off_old)
-        return launch_scan_tiles(stream, count.as<uint32_t>(), off, tile_totals(), ntiles,
+        return launch_scan_tiles(stream, count.as<uint32_t>(), off, tile_totals(), inc_tiles(),
                                  ovf_ctr.as<uint32_t>(), flags.as<uint32_t>(), (uint32_t)n);
       if (ntiles_direct)
         return launch_scan_direct(stream, count.as<uint32_t>(), off, tiles(), ovf_ctr.as<uint32_t>(),
