@@ -231,11 +231,8 @@ def bench_batched(steps: int = 128, warmup: int = 10, graph: bool = True) -> dic
     cells = b.num_cells
     b.close()
     sims = [psx.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) for sc in scs]
-    # the per-scene handles take the batch's launch shapes with the id slots of sort_variant 1: the
-    # small grids among them keep the direct scan, which the incremental form gives up
     for s in sims:
-        s.set_tuning(**{**tuning, "use_graph": 1 if graph else 0,
-                        "sort_variant": 1 if tuning.get("sort_variant") == 3 else tuning.get("sort_variant", 0)})
+        s.set_tuning(**{**tuning, "use_graph": 1 if graph else 0})
         s.step(warmup)
     for s in sims:
         s.synchronize()

@@ -100,10 +100,10 @@ typedef struct psx_tuning {
   int32_t sort_variant;     /* cell lists of the bucketing: 1 per-cell id slots (three kernels per
                                step in a run: scan, rank/gather, pairs), 2 dense lists filled by a
                                scatter kernel (four), 0 automatic (id slots), 3 incremental id
-                               slots: counts kept from step to step, only the spheres that change
-                               cell update them, with the look-back scan on every grid (with
-                               PSX_FLAG_SEMANTICS_CUDA as 1). Results are the same for every
-                               value. */
+                               slots: counts and per-tile totals kept from step to step, only the
+                               spheres that change cell update them, and a scan from the kept tile
+                               totals (with PSX_FLAG_SEMANTICS_CUDA as 1). Results are the same for
+                               every value. */
   int32_t pair_chunk;       /* candidates per round of the one-lane pair kernel (pair_variant 0 or 1):
                                0 or 4, or 2 (fewer wasted tests where column slices hold few spheres) */
 } psx_tuning;

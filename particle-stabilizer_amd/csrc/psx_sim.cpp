@@ -293,7 +293,6 @@ struct SimImpl final : SimBase {
   DevBuf cell_of, cell_ids, ovf, ovf_ctr, home, list, stat_part;
   DevBuf tile_count, scan_gen;  // direct scan: per-tile counts (two generations), generation
   int32_t ntiles_direct = 0;    // tiles of the direct scan, 0: look-back scan
-  int32_t ntiles_direct_grid = 0;  // tiles of the direct scan when the grid and the form allow it
   DevBuf count, offset, tile_state, flags;
   // incremental bucketing (sort_variant 3): per-cell arrival words and change marks, the second
   // offset array (the offsets alternate with the position buffers), the bucket cells of
@@ -489,8 +488,7 @@ struct SimImpl final : SimBase {
     cpad = (cells + kScanTile - 1) / kScanTile * kScanTile;
     ntiles = (int32_t)(cpad / kScanTile);
     id_stride = (cells + kIdPlane - 1) / kIdPlane * kIdPlane;
-    ntiles_direct_grid = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
-    ntiles_direct = incremental() ? 0 : ntiles_direct_grid;
+    ntiles_direct = cpad / kDirectTile <= kDirectTiles ? (int32_t)(cpad / kDirectTile) : 0;
 
     // device allocations
     for (int k = 0; k < 2; ++k) {
@@ -507,7 +505,7 @@ struct SimImpl final : SimBase {
     count.alloc((size_t)cpad * 4);
     offset.alloc(((size_t)cpad + 1 + kOffsetPad) * 4);
     tile_state.alloc((size_t)ntiles * 8);
-    tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct_grid * kTileStride, 1) * 4);
+    tile_count.alloc(std::max<size_t>(2 * (size_t)ntiles_direct * kTileStride, 1) * 4);
     scan_gen.alloc(4);
     flags.alloc(16);
     io.alloc((size_t)nn * 6 * sizeof(T));
@@ -553,10 +551,12 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipDeviceSynchronize());
   }
 
-  // Incremental bucketing (sort_variant 3): kept counts, atomics only for spheres that change cell
-  // (NextBucket, IncRank). It takes the look-back scan on every grid (the direct scan's per-tile
-  // counts are rebuilt by every hash pass); the CUDA backend's step uses the id slots of
-  // sort_variant 1.
+  // Incremental bucketing (sort_variant 3): kept counts and tile totals, atomics only for spheres
+  // that change cell (NextBucket, IncRank), and the scan from the kept totals (k_scan_tiles) in place
+  // of the direct or look-back scan, whose bookkeeping every hash pass rebuilds; the full form's
+  // bucketings of the same handle (statistics, windows) keep the grid's scan. Every change between
+  // the two goes through drop_hash, which clears the direct scan's per-tile counts. The CUDA
+  // backend's step uses the id slots of sort_variant 1.
   bool incremental() const { return tuning.sort_variant == 3 && !cuda; }
 
   // The buffers of the incremental bucketing, allocated when the tuning asks for it (at set-up or
@@ -577,7 +577,7 @@ struct SimImpl final : SimBase {
 
   // Tiles of the incremental bucketing's scan: kDirectTile cells on grids the direct scan covers
   // (more workgroups than 16,384-cell tiles would give), kScanTile cells on larger ones.
-  int32_t inc_shift() const { return ntiles_direct_grid ? kDirectShift : kScanShift; }
+  int32_t inc_shift() const { return ntiles_direct ? kDirectShift : kScanShift; }
   int32_t inc_tiles() const { return (int32_t)(cpad >> inc_shift()); }
 
   TileTotals tile_totals() const {
@@ -615,9 +615,11 @@ struct SimImpl final : SimBase {
   // 4.6M spheres, 4e5 to 3.5e7 cells), so they are the automatic choice.
   bool id_blocks() const { return tuning.sort_variant != 2; }
 
-  TileCounts tiles() const {
+  // The direct scan's per-tile counts for a hash pass; none for the incremental bucketing's passes,
+  // whose scan keeps tile totals of its own (TileTotals).
+  TileCounts tiles(bool inc = false) const {
     TileCounts t;
-    t.tc = ntiles_direct ? tile_count.as<uint32_t>() : nullptr;
+    t.tc = (ntiles_direct && !inc) ? tile_count.as<uint32_t>() : nullptr;
     t.gen = scan_gen.as<uint32_t>();
     t.ntiles = ntiles_direct;
     return t;
@@ -644,7 +646,7 @@ struct SimImpl final : SimBase {
                                  cell_of.as<int32_t>(), count.as<uint32_t>(),
                                  cell_ids.as<int32_t>(), (uint32_t)id_stride, ovf.as<int2>(),
                                  ovf_ctr.as<uint32_t>(),
-                                 tiles(), mode, inc ? arr.as<uint64_t>() : nullptr,
+                                 tiles(inc), mode, inc ? arr.as<uint64_t>() : nullptr,
                                  inc ? mark.as<uint8_t>() : nullptr);
     });
   }
@@ -732,7 +734,7 @@ struct SimImpl final : SimBase {
       nb.ovf = ovf.as<int2>();
       nb.ovf_ctr = ovf_ctr.as<uint32_t>();
       nb.blocks = id_blocks() ? 1 : 0;
-      nb.tiles = tiles();
+      nb.tiles = tiles(inc);
       if (inc) {
         nb.arr = arr.as<unsigned long long>();
         nb.mark = mark.as<uint8_t>();
@@ -789,7 +791,7 @@ struct SimImpl final : SimBase {
     if (!hashed) return;
     PSX_HIP(launch_zero_u32(stream, count.as<uint32_t>(), (int64_t)cpad));
     PSX_HIP(launch_zero_u32(stream, ovf_ctr.as<uint32_t>(), 1));
-    PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct_grid * kTileStride));
+    PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct * kTileStride));
     hashed = false;
   }
 
@@ -1187,12 +1189,6 @@ struct SimImpl final : SimBase {
     PSX_HIP(hipStreamSynchronize(stream));
     destroy_graphs();
     tuning = t;
-    const int32_t was = ntiles_direct;
-    ntiles_direct = incremental() ? 0 : ntiles_direct_grid;
-    if (ntiles_direct != was) {  // the per-tile counts of both generations start again from zero
-      PSX_HIP(launch_zero_u32(stream, tile_count.as<uint32_t>(), 2 * (int64_t)ntiles_direct_grid * kTileStride));
-      PSX_HIP(hipStreamSynchronize(stream));
-    }
     int nb;
     table(tuning.block_integrate, &nb);
     table(tuning.block_sort, &nb);
