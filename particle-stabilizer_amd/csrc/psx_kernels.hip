@@ -311,15 +311,16 @@ __device__ __forceinline__ void integrate_velocity(const SceneDev<T>& s, Vec4<T>
 // from the state the next step starts from. Returns the cell (tile counts are added per workgroup
 // by WgTiles).
 // With the incremental bucketing only a particle whose next cell differs from its bucket cell
-// counts and files itself (bucket_move); every particle writes its next cell.
+// counts and files itself (bucket_move); every particle writes its next cell. hw: the particle's
+// home word (nb.home[p]), which the pair kernels have loaded for their walk.
 template <class T>
 __device__ __forceinline__ int32_t next_bucket(const SceneDev<T>& s, const NextBucket& nb,
-                                               Vec4<T> x, const Vec4<T>& v, int32_t p) {
+                                               Vec4<T> x, const Vec4<T>& v, int32_t p,
+                                               uint32_t hw) {
   integrate_position(s, x, v);
   if (nb.arr) {
     const int32_t c = grid_cell(s, x);
     nb.cell_of[p] = c;
-    const uint32_t hw = nb.home[p];
     const int32_t c_old = (hw & kHomeMoved) ? nb.cell_cur[p] : (int32_t)(hw & kHomeCell);
     if (c != c_old)
       bucket_move(nb.id[p], c, c_old, nb.ovf_ctr[2] + 1u, nb.count, nb.arr, nb.mark, nb.tt,
@@ -1586,10 +1587,12 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
       PairSelf<T> me;
       NearQueue nq{near_raw + threadIdx.x, blockDim.x, 0u, ~0u};
       T ax, ay, az;
+      uint32_t hw_next = 0;  // the home word for the incremental hash pass
       if constexpr (W == 2 && sizeof(T) == 4) {
         me.x = pos[p];
         me.v = vsn[p];
         const uint32_t hw = home[p];
+        hw_next = hw;
         const int32_t hc = (int32_t)(hw & kHomeCell);
         const T rr = me.x.w + sc.rmax;
         me.thr = (rr * rr) * PairMargin<T>::value;
@@ -1615,6 +1618,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
       } else {
         uint32_t lo[9], hi[9];
         pair_setup(&sc, pos, vsn, offset, home, p, me, lo, hi);
+        if (next.arr) hw_next = home[p];
         ax = me.v.x;
         ay = me.v.y;
         az = me.v.z;
@@ -1628,7 +1632,7 @@ __global__ void __launch_bounds__(1024, (W == 2 && sizeof(T) == 4) ? 8 : 1)
       }
       const Vec4<T> vn{ax, ay, az, me.v.w};
       vout[p] = vn;
-      if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+      if (next.count) c = next_bucket(sc, next, me.x, vn, p, hw_next);
     }
     if (tiles) {
       wt.add(next.tiles, c);
@@ -1687,7 +1691,7 @@ __global__ void __launch_bounds__(1024)
       if (g == 0) {
         const Vec4<T> vn{ax, ay, az, me.v.w};
         vout[p] = vn;
-        if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+        if (next.count) c = next_bucket(sc, next, me.x, vn, p, next.arr ? home[p] : 0u);
       }
     }
     if (tiles) {
@@ -1792,7 +1796,7 @@ __global__ void __launch_bounds__(512)
       }
       const Vec4<T> vn{ax, ay, az, me.v.w};
       vout[p] = vn;
-      if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+      if (next.count) c = next_bucket(sc, next, me.x, vn, p, next.arr ? home[p] : 0u);
     }
     if (tiles) {
       wt.add(next.tiles, c);
@@ -2251,7 +2255,7 @@ __global__ void __launch_bounds__(1024)
       if (active) {
         const Vec4<T> vn{ax, ay, az, me.v.w};
         vout[p] = vn;
-        if (next.count) c = next_bucket(sc, next, me.x, vn, p);
+        if (next.count) c = next_bucket(sc, next, me.x, vn, p, next.arr ? home[p] : 0u);
       }
     }
     if (tiles) {
