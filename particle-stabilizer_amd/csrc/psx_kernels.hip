@@ -869,21 +869,29 @@ __global__ void __launch_bounds__(kDirectThreads)
 // members, from the cell's chain of overflow entries (count - kCellIds of them), so the cost of a
 // particle is bounded by its own cell's occupancy.
 // ---------------------------------------------------------------------------------------------
+// The rank a sphere keeps in a cell nothing entered or left: the byte the last ranking stored for
+// its slot (r_old), or its place in the cell's old range where that byte is saturated.
+__device__ __forceinline__ uint32_t inc_kept_rank(const IncRank& inc, int32_t c, uint32_t p,
+                                                  uint32_t r_old) {
+  return r_old < 255u ? r_old : p - inc.off_old[c];
+}
+
 // The rank of particle me (old sorted slot p, bucket cell c) in the incremental bucketing (IncRank):
-// its place in the cell's old range when the cell's mark or, where the mark matches, its arrival
-// word is of an earlier generation (nothing entered or left it), else the members of the old range that stay in c and the cell's
-// arrivals (id slots, then the overflow chain), counted where their id is smaller. The range is
-// clamped to the particles, so stale offsets cannot send the loop past the arrays.
+// the rank it keeps (inc_kept_rank) when the cell's mark or, where the mark matches, its arrival
+// word is of an earlier generation (nothing entered or left the cell), else the members of the
+// cell's old range that stay in c and the cell's arrivals (id slots, then the overflow chain),
+// counted where their id is smaller. The range is clamped to the particles, so stale offsets cannot
+// send the loop past the arrays.
 __device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, int32_t c, int32_t me,
-                                             uint32_t p, uint32_t n,
+                                             uint32_t p, uint32_t r_old, uint32_t n,
                                              const int32_t* __restrict__ cell_of,
                                              const int32_t* __restrict__ id,
                                              const int32_t* __restrict__ cell_ids,
                                              uint32_t id_stride, const int2* __restrict__ ovf) {
-  const uint32_t lo = inc.off_old[c];
-  if (inc.mark[c] != (uint8_t)gen) return p - lo;
+  if (inc.mark[c] != (uint8_t)gen) return inc_kept_rank(inc, c, p, r_old);
   const uint64_t w = inc.arr[c];
-  if ((uint32_t)(w >> 32) != gen) return p - lo;
+  if ((uint32_t)(w >> 32) != gen) return inc_kept_rank(inc, c, p, r_old);
+  const uint32_t lo = inc.off_old[c];
   const uint32_t hi = min(inc.off_old[c + 1], n);
   uint32_t rank = 0;
   for (uint32_t q = min(lo, hi); q < hi; ++q)
@@ -933,6 +941,8 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
     if (active) {
       c = cell_of[p];
       me = id[p];
+      // the slot's kept rank, loaded with the slot's other words (a cell's mark decides its use)
+      const uint32_t r_old = (kBlocks && inc.arr) ? inc.rank_old[p] : 0u;
       x = pos[p];
       v = vel[p];
       if (mode & kModeIntegrate) {
@@ -949,8 +959,8 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       uint32_t rank = 0;
       if constexpr (kBlocks) {
         if (inc.arr) {
-          rank = inc_rank(inc, inc_gen, c, me, (uint32_t)p, (uint32_t)n, cell_of, id, cell_ids,
-                          id_stride, ovf);
+          rank = inc_rank(inc, inc_gen, c, me, (uint32_t)p, r_old, (uint32_t)n, cell_of, id,
+                          cell_ids, id_stride, ovf);
         } else {
         const uint32_t cnt = e - b;
         if (cnt > 1u) {  // a particle alone in its cell has rank 0
@@ -976,6 +986,9 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       // b + rank < n whenever the offsets are the exact scan; the bound keeps the store inside the
       // arrays even if a stalled scan (reported through the flags) left them short
       dst = min(b + rank, (uint32_t)(n - 1));
+      if constexpr (kBlocks) {
+        if (inc.arr) inc.rank_o[dst] = (uint8_t)min(rank, 255u);
+      }
     }
     // The container acts on the pre-container state the bucketing hashed; its triangle-list loads
     // run alongside the rank's offset and id-block loads.
@@ -2128,7 +2141,8 @@ hipError_t launch_rank_gather(hipStream_t st, int ntab, int block, const BlockIn
                               uint32_t* gen, const IncRank& inc) {
   const Locator<T> loc = make_locator(ntab, block, btab, scenes, one);
   const dim3 grid(xcd_grid(ntab)), blk(block);
-  if (inc.arr && (!blocks || !inc.off_old || !inc.cell_cur_o || !inc.mark)) return hipErrorInvalidValue;
+  if (inc.arr && (!blocks || !inc.off_old || !inc.cell_cur_o || !inc.mark || !inc.rank_old || !inc.rank_o))
+    return hipErrorInvalidValue;
 #define PSX_RANK(TABLE, BLOCKS)                                                                  \
   PSX_LAUNCH((k_rank_gather<T, TABLE, BLOCKS>), grid, blk, 0, st, loc, cell_of, offset,         \
              cell_ids, id_stride, ovf, ovf_ctr, list, pos, vel, id, tent, tcell_off, flags, mode,\

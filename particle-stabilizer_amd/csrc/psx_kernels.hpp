@@ -197,12 +197,16 @@ hipError_t launch_scan_direct(hipStream_t st, uint32_t* count, uint32_t* offset,
 // cell c). A particle whose cell nothing entered or left keeps its place in that range; in any other
 // cell it counts the cell's remaining members with a smaller id (old members whose new cell is still
 // the cell) and its arrivals with a smaller id. cell_cur_o receives the bucket cell of every sorted
-// slot whose home cell differs from it (kHomeMoved).
+// slot whose home cell differs from it (kHomeMoved). rank_o receives every sorted slot's rank (255
+// for 255 and more) and rank_old holds those of the state being ranked, where the place of a
+// sphere of an unchanged cell is read (one byte per sphere instead of the cell's old offset).
 struct IncRank {
   const uint64_t* arr = nullptr;
   const uint8_t* mark = nullptr;
   const uint32_t* off_old = nullptr;
   int32_t* cell_cur_o = nullptr;
+  const uint8_t* rank_old = nullptr;
+  uint8_t* rank_o = nullptr;
 };
 
 // Stable rank among cell mates (from the id blocks, or from the dense lists when !blocks); the

@@ -297,7 +297,7 @@ struct SimImpl final : SimBase {
   // incremental bucketing (sort_variant 3): per-cell arrival words and change marks, the second
   // offset array (the offsets alternate with the position buffers), the bucket cells of
   // container-moved spheres and the kept per-tile totals of the counts (two generations)
-  DevBuf arr, mark, offset2, cell_cur, tile_tot;
+  DevBuf arr, mark, offset2, cell_cur, tile_tot, rank8[2];
   DevBuf scenes_d, tgeo_d, tbox_d, tent_d, tcell_off_d, cull_d;
   std::vector<SceneDev<T>> scenes_h;  // host copy of scenes_d
   DevBuf io;  // 6n T staging for state transfer
@@ -569,6 +569,7 @@ struct SimImpl final : SimBase {
     offset2.alloc(offset.bytes);
     cell_cur.alloc((size_t)n * 4);
     tile_tot.alloc((size_t)2 * inc_tiles() * 4);
+    for (DevBuf& r : rank8) r.alloc((size_t)n);
     PSX_HIP(hipMemset(arr.p, 0, arr.bytes));
     PSX_HIP(hipMemset(mark.p, 0, mark.bytes));
     PSX_HIP(hipMemset(offset2.p, 0, offset2.bytes));
@@ -652,11 +653,12 @@ struct SimImpl final : SimBase {
   }
 
   // Ordering after a hash pass: offsets (clearing the counts), lists, rank/gather into pd/vd/idd.
-  // off_old: the incremental bucketing's order, from the offsets of the state (ps, vs, is); its
-  // counts are kept.
+  // off_old: the incremental bucketing's order, from the offsets of the state (ps, vs, is) and its
+  // slots' ranks (rank_old), writing the ranks of the sorted state to rank_o; its counts are kept.
   void order(const Vec4<T>* ps, const Vec4<T>* vs, const int32_t* is, Vec4<T>* pd, Vec4<T>* vd,
              int32_t* idd, uint32_t* off, int32_t* cell_sorted, int mode, Profiler* prof,
-             Vec4<T>* xpre_sorted, const uint32_t* off_old = nullptr) {
+             Vec4<T>* xpre_sorted, const uint32_t* off_old = nullptr,
+             const uint8_t* rank_old = nullptr, uint8_t* rank_o = nullptr) {
     int nb_s;
     const BlockInfo* bt_s = table(tuning.block_sort, &nb_s);
     const SceneDev<T>* one = single() ? scenes_h.data() : nullptr;
@@ -668,6 +670,8 @@ struct SimImpl final : SimBase {
       inc.mark = mark.as<uint8_t>();
       inc.off_old = off_old;
       inc.cell_cur_o = cell_cur.as<int32_t>();
+      inc.rank_old = rank_old;
+      inc.rank_o = rank_o;
     }
     launch(1, prof, [&] {
       if (off_old)
@@ -828,12 +832,13 @@ struct SimImpl final : SimBase {
     const int a = cur_pos, vc = cur_vel;
     ensure_hashed(prof);
     const bool inc = incremental();
-    // the incremental bucketing reads the offsets of the state it orders (position buffer a) and
-    // writes those of the sorted state into the other offset array
+    // the incremental bucketing reads the offsets and slot ranks of the state it orders (position
+    // buffer a) and writes those of the sorted state into the other arrays
     uint32_t* const off = inc ? offs_at(1 - a) : offs();
     order(pos[a].as<Vec4<T>>(), vel[vc].as<Vec4<T>>(), id[a].as<int32_t>(), pos[1 - a].as<Vec4<T>>(),
           vel[1 - vc].as<Vec4<T>>(), id[1 - a].as<int32_t>(), off, nullptr,
-          kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr, inc ? offs_at(a) : nullptr);
+          kModeIntegrate | kModeContainer | kModeCheck, prof, nullptr, inc ? offs_at(a) : nullptr,
+          rank8[a].as<uint8_t>(), rank8[1 - a].as<uint8_t>());
     pairs(pos[1 - a].as<Vec4<T>>(), vel[1 - vc].as<Vec4<T>>(), off,
           vel[vc].as<Vec4<T>>(), prof, id[1 - a].as<int32_t>(), inc);
     cur_pos = 1 - a;
