@@ -53,14 +53,14 @@ def algorithmic_bytes(n: int, cells: int, s: int, form: int) -> dict:
     (a few per thousand in a settle) being left out; k_integrate only starts a run. The pair kick
     itself reads the sorted state (8 s N) and the cell offsets (8 C) and writes the new velocities
     (3 s N). The incremental form's scan leaves the counts alone (8 C instead of 12 C) and its rank
-    kernel reads each sphere's cell mark and old offset (5 N more)."""
+    kernel reads each sphere's cell mark and kept rank and writes its new rank (3 N more)."""
     blocks = form != 2
     inc = form == 3
     return {
         "k_integrate": n * (13 * s + 16),
         "k_scan": (12 if blocks and not inc else 8) * cells,
         "k_scatter_ids": 0 if blocks else 16 * n + 8 * cells,
-        "k_rank_gather": n * (16 * s + 24 + (5 if inc else 0)),
+        "k_rank_gather": n * (16 * s + 24 + (3 if inc else 0)),
         "k_pairs": 11 * s * n + 8 * cells + (4 * n if inc else n * (12 if blocks else 8) + 8 * n),
     }
 

@@ -34,11 +34,12 @@ def test_pair_kernel_bytes_of_the_hopper():
     assert bench.algorithmic_bytes(n, c, 4, 1)["k_pairs"] == 64 * n + 8 * c == 360200880
     assert bench.algorithmic_bytes(n, c, 4, 0) == bench.algorithmic_bytes(n, c, 4, 1)
     # the incremental form: the hash pass writes the cells only (4 N), the scan reads the counts
-    # and writes the offsets (8 C), the rank kernel reads a mark and an old offset per sphere
+    # and writes the offsets (8 C), the rank kernel reads a mark and a kept rank and writes a rank
+    # per sphere (one byte each)
     inc = bench.algorithmic_bytes(n, c, 4, 3)
     assert inc["k_pairs"] == 48 * n + 8 * c == 293092016
     assert inc["k_scan"] == 8 * c
-    assert inc["k_rank_gather"] == bench.algorithmic_bytes(n, c, 4, 1)["k_rank_gather"] + 5 * n
+    assert inc["k_rank_gather"] == bench.algorithmic_bytes(n, c, 4, 1)["k_rank_gather"] + 3 * n
 
 
 def test_cpu_threads_follow_the_cpu_share(monkeypatch):
