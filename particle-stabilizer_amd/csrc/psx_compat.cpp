@@ -95,6 +95,15 @@ extern "C" void runSimulateHIP(const std::vector<psxcompat::ParticlePtr>& partic
                  containerPath.empty() ? nullptr : &mesh, &sim) != PSX_OK)
     critical("psx_create");
   psx_free(tri);
+  {
+    // steps replayed from hipGraphs and the incremental bucketing: the same results as every other
+    // launch shape, faster on every workload measured (the shapes themselves keep their defaults)
+    psx_tuning t;
+    if (psx_get_tuning(sim, &t) != PSX_OK) critical("psx_get_tuning");
+    t.use_graph = 1;
+    t.sort_variant = 3;
+    if (psx_set_tuning(sim, &t) != PSX_OK) critical("psx_set_tuning");
+  }
 
   {
     double mn[3], h;
