@@ -717,9 +717,9 @@ __global__ void __launch_bounds__(kScanThreads)
 // waits for another. Each tile writes its own total, counted from its cells, into the other
 // generation's entry for the next bucketing and raises kFlagCountDrift where the kept total
 // differs from it. Offsets are clamped to `limit` as in k_scan; the counts are left in place.
-// Tiles of kRounds sub-tiles as in k_tile_totals: kScanTile cells on large grids, kDirectTile cells
-// on grids of at most kDirectTiles of them (more workgroups, each summing at most kDirectTiles
-// totals).
+// Tiles of kRounds sub-tiles as in k_tile_totals: kDirectTile cells (one round) on grids of at most
+// kDirectTiles of them, 4,096 (four rounds) on grids of at most 4,096 of those, kScanTile cells (16)
+// above: more workgroups where a workgroup's sum of the totals before its tile stays short.
 template <int kRounds>
 __global__ void __launch_bounds__(kScanThreads)
     k_scan_tiles(const uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
@@ -2099,6 +2099,8 @@ hipError_t launch_scan(hipStream_t st, uint32_t* count, uint32_t* offset, uint64
 hipError_t launch_tile_totals(hipStream_t st, const uint32_t* count, const TileTotals& tt) {
   if (tt.shift == kDirectShift)
     PSX_LAUNCH(k_tile_totals<1>, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
+  else if (tt.shift == kDirectShift + 2)
+    PSX_LAUNCH(k_tile_totals<4>, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
   else if (tt.shift == kScanShift)
     PSX_LAUNCH(k_tile_totals<kScanPerThread / 4>, dim3(tt.ntiles), dim3(kScanThreads), 0, st, count, tt);
   else
@@ -2111,6 +2113,9 @@ hipError_t launch_scan_tiles(hipStream_t st, const uint32_t* count, uint32_t* of
                              uint32_t n) {
   if (tt.shift == kDirectShift)
     PSX_LAUNCH(k_scan_tiles<1>, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tt, ntiles,
+               ovf_ctr, flags, n);
+  else if (tt.shift == kDirectShift + 2)
+    PSX_LAUNCH(k_scan_tiles<4>, dim3(ntiles), dim3(kScanThreads), 0, st, count, offset, tt, ntiles,
                ovf_ctr, flags, n);
   else if (tt.shift == kScanShift)
     PSX_LAUNCH(k_scan_tiles<kScanPerThread / 4>, dim3(ntiles), dim3(kScanThreads), 0, st, count,

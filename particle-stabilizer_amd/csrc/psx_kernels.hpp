@@ -91,7 +91,7 @@ struct TileTotals {
   uint32_t* tot = nullptr;
   const uint32_t* gen = nullptr;
   int32_t ntiles = 0;
-  int32_t shift = 0;  // log2 of the cells per tile: kScanShift, or kDirectShift on small grids
+  int32_t shift = 0;  // log2 of the cells per tile: kDirectShift, kDirectShift + 2 or kScanShift
 };
 
 // Incremental bucketing (arr != nullptr; look-back-scan grids, id slots): the counts persist from
