@@ -135,9 +135,11 @@ int psx_destroy(psx_sim* sim);
 int psx_step(psx_sim* sim, int64_t nsteps);
 
 /* Captures the hipGraphs that psx_step(sim, nsteps) would replay from the current state, without
- * running a step (the current state's bucketing hash pass is enqueued, as psx_step would). A run
- * whose graphs are prepared spends no time capturing or instantiating them. No counterpart in the
- * reference (its loop launches every kernel from the host, PhysicsEngineCUDA.cpp:25-81). */
+ * running a step (the current state's bucketing hash pass is enqueued, as psx_step would): 64- and
+ * 16-step graphs, and one of the 1-15 steps left after them, which psx_step replays only where it
+ * was prepared (otherwise those steps are launched one by one). A run whose graphs are prepared
+ * spends no time capturing or instantiating them and launches no step on its own. No counterpart in
+ * the reference (its loop launches every kernel from the host, PhysicsEngineCUDA.cpp:25-81). */
 int psx_prepare(psx_sim* sim, int64_t nsteps);
 
 int psx_synchronize(psx_sim* sim);

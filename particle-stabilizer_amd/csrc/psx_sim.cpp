@@ -310,6 +310,10 @@ struct SimImpl final : SimBase {
   // runs on graph replay.
   static constexpr int kGraphSteps[2] = {64, 16};
   hipGraphExec_t graphs[2][2][2] = {};  // [length][position buffer][velocity buffer]
+  // Graphs of the 1-15 steps a run leaves after its 64- and 16-step chunks, captured by prepare()
+  // only (a run that was not prepared takes them as single steps): [steps][position][velocity]. An
+  // odd-length graph ends on the other position buffer.
+  hipGraphExec_t tail_graphs[16][2][2] = {};
 
   ~SimImpl() override {
     destroy_graphs();
@@ -319,6 +323,13 @@ struct SimImpl final : SimBase {
   void destroy_graphs() {
     destroy_window_graphs();
     for (auto& len : graphs)
+      for (auto& row : len)
+        for (auto& g : row)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+    for (auto& len : tail_graphs)
       for (auto& row : len)
         for (auto& g : row)
           if (g) {
@@ -1130,11 +1141,12 @@ struct SimImpl final : SimBase {
     d.ew = (T)prm.walls[0];
   }
 
-  void capture(int li, int a, int vc) {
+  // Captures nsteps steps from buffer parity (a, vc) into *out; the host state is left as it was.
+  void capture(int nsteps, int a, int vc, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     PSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < kGraphSteps[li]; ++k) step_once(nullptr);
+      for (int k = 0; k < nsteps; ++k) step_once(nullptr);
     } catch (...) {
       hipGraph_t dummy = nullptr;
       (void)hipStreamEndCapture(stream, &dummy);
@@ -1144,9 +1156,11 @@ struct SimImpl final : SimBase {
       throw;
     }
     PSX_HIP(hipStreamEndCapture(stream, &g));
-    hipError_t e = hipGraphInstantiate(&graphs[li][a][vc], g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) throw Error(PSX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // the graph's device-side set-up now, in stream order, rather than at its first launch
+    PSX_HIP(hipGraphUpload(*out, stream));
     cur_pos = a;
     cur_vel = vc;
   }
@@ -1157,25 +1171,34 @@ struct SimImpl final : SimBase {
     if (tuning.use_graph) {
       for (int li = 0; li < 2; ++li) {
         while (left >= kGraphSteps[li]) {
-          if (!graphs[li][cur_pos][cur_vel]) capture(li, cur_pos, cur_vel);
-          PSX_HIP(hipGraphLaunch(graphs[li][cur_pos][cur_vel], stream));
+          hipGraphExec_t& g = graphs[li][cur_pos][cur_vel];
+          if (!g) capture(kGraphSteps[li], cur_pos, cur_vel, &g);
+          PSX_HIP(hipGraphLaunch(g, stream));
           left -= kGraphSteps[li];
         }
+      }
+      if (left > 0 && tail_graphs[left][cur_pos][cur_vel]) {  // prepared
+        PSX_HIP(hipGraphLaunch(tail_graphs[left][cur_pos][cur_vel], stream));
+        cur_pos ^= (int)(left & 1);
+        left = 0;
       }
     }
     for (; left > 0; --left) step_once(nullptr);
   }
 
-  // A run of nsteps from the current state replays graphs of the current buffer parity only (both
-  // graph lengths are even): capture those it will use.
+  // A run of nsteps from the current state replays 64- and 16-step graphs of the current buffer
+  // parity (both lengths are even) and then the graph of the steps left: capture those it will use.
   void prepare(int64_t nsteps) override {
     if (nsteps <= 0 || !tuning.use_graph) return;
     ensure_hashed(nullptr);
     int64_t left = nsteps;
     for (int li = 0; li < 2; ++li) {
-      if (left >= kGraphSteps[li] && !graphs[li][cur_pos][cur_vel]) capture(li, cur_pos, cur_vel);
+      hipGraphExec_t& g = graphs[li][cur_pos][cur_vel];
+      if (left >= kGraphSteps[li] && !g) capture(kGraphSteps[li], cur_pos, cur_vel, &g);
       left %= kGraphSteps[li];
     }
+    hipGraphExec_t& t = tail_graphs[left][cur_pos][cur_vel];
+    if (left > 0 && !t) capture((int)left, cur_pos, cur_vel, &t);
   }
 
   void profile(int64_t nsteps, double* ms, double* per_step) override {

@@ -345,3 +345,29 @@ def test_large_grid_steps_bitexact(psx_mod, oracle, dtype):
             p, v = hip_run(sc, 20, graph=graph, sort_variant=variant)
             assert_bitwise(as_dtype(p, dtype), ref_p, f"pos sort_variant={variant} graph={graph}")
             assert_bitwise(as_dtype(v, dtype), ref_v, f"vel sort_variant={variant} graph={graph}")
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("form", [1, 3])
+def test_prepared_tail_graphs_equal_eager(psx_mod, oracle, dtype, form):
+    """Runs of 1, 7, 31 (16 + 15), 69 (64 + 5), 7 and 1 steps, each prepared first, so that its last
+    1-15 steps replay a graph of that length (an odd length ends on the other position buffer, and the
+    next run starts from it): every run's state equals an eager handle's stepped alongside, bit for
+    bit, and the whole run equals the oracle's."""
+    sc = dense_box(dtype=dtype)
+    runs = (1, 7, 31, 69, 7, 1)
+    with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as g, \
+            psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params) as e:
+        g.set_tuning(use_graph=1, sort_variant=form)
+        e.set_tuning(use_graph=0, sort_variant=form)
+        for r, k in enumerate(runs):
+            g.prepare(k)
+            g.step(k)
+            e.step(k)
+            pg, vg = g.state()
+            pe, ve = e.state()
+            assert_bitwise(pg, pe, f"pos run {r} ({k} steps)")
+            assert_bitwise(vg, ve, f"vel run {r} ({k} steps)")
+    ref_p, ref_v = oracle_run(oracle, sc, sum(runs))
+    assert_bitwise(as_dtype(pg, dtype), ref_p, "pos vs oracle")
+    assert_bitwise(as_dtype(vg, dtype), ref_v, "vel vs oracle")
