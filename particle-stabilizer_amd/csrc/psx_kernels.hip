@@ -877,18 +877,18 @@ __device__ __forceinline__ uint32_t inc_kept_rank(const IncRank& inc, int32_t c,
 }
 
 // The rank of particle me (old sorted slot p, bucket cell c) in the incremental bucketing (IncRank):
-// the rank it keeps (inc_kept_rank) when the cell's mark or, where the mark matches, its arrival
+// the rank it keeps (inc_kept_rank) when the cell's mark (mk) or, where the mark matches, its arrival
 // word is of an earlier generation (nothing entered or left the cell), else the members of the
 // cell's old range that stay in c and the cell's arrivals (id slots, then the overflow chain),
 // counted where their id is smaller. The range is clamped to the particles, so stale offsets cannot
 // send the loop past the arrays.
 __device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, int32_t c, int32_t me,
-                                             uint32_t p, uint32_t r_old, uint32_t n,
+                                             uint32_t p, uint32_t r_old, uint8_t mk, uint32_t n,
                                              const int32_t* __restrict__ cell_of,
                                              const int32_t* __restrict__ id,
                                              const int32_t* __restrict__ cell_ids,
                                              uint32_t id_stride, const int2* __restrict__ ovf) {
-  if (inc.mark[c] != (uint8_t)gen) return inc_kept_rank(inc, c, p, r_old);
+  if (mk != (uint8_t)gen) return inc_kept_rank(inc, c, p, r_old);
   const uint64_t w = inc.arr[c];
   if ((uint32_t)(w >> 32) != gen) return inc_kept_rank(inc, c, p, r_old);
   const uint32_t lo = inc.off_old[c];
@@ -945,21 +945,24 @@ __global__ void __launch_bounds__(kMaxBlockSort<T>)
       const uint32_t r_old = (kBlocks && inc.arr) ? inc.rank_old[p] : 0u;
       x = pos[p];
       v = vel[p];
+      // the cell's words, all issued once the cell is known and before anything waits for the
+      // particle's state (the integration) or for the flag byte (the triangle-list branch)
+      cull_c = cull[c];
+      const uint32_t b = offset[c], e = offset[c + 1];
+      const uint8_t mk = (kBlocks && inc.arr) ? inc.mark[c] : (uint8_t)0;
       if (mode & kModeIntegrate) {
         integrate_position(s, x, v);
         integrate_velocity(s, v);
       }
-      cull_c = cull[c];
       if (mesh && (cull_c & kCellTris)) {  // most cells list no triangle: their bounds go unread
         const int32_t tc = s.tcell_base + (c - s.grid.cell_base);
         k0 = tcell_off[tc];
         k1 = tcell_off[tc + 1];
       }
-      const uint32_t b = offset[c], e = offset[c + 1];
       uint32_t rank = 0;
       if constexpr (kBlocks) {
         if (inc.arr) {
-          rank = inc_rank(inc, inc_gen, c, me, (uint32_t)p, r_old, (uint32_t)n, cell_of, id,
+          rank = inc_rank(inc, inc_gen, c, me, (uint32_t)p, r_old, mk, (uint32_t)n, cell_of, id,
                           cell_ids, id_stride, ovf);
         } else {
         const uint32_t cnt = e - b;
