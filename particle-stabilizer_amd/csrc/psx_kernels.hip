@@ -911,8 +911,12 @@ __device__ __forceinline__ uint32_t inc_rank(const IncRank& inc, uint32_t gen, i
   return rank;
 }
 
+// The binary32 form is held to seven waves per SIMD (at most 72 VGPRs, no spills; 79 unbounded, six
+// waves): its waves wait on memory most of their cycles, and one more resident wave took the hopper's
+// rank kernel from 78.0 to 77.0 us (same box; held to 64 VGPRs, eight waves, it spills in the mesh
+// path and runs slower).
 template <class T, bool kTable, bool kBlocks>
-__global__ void __launch_bounds__(kMaxBlockSort<T>)
+__global__ void __launch_bounds__(kMaxBlockSort<T>, sizeof(T) == 4 ? 7 : 1)
     k_rank_gather(const Locator<T> loc, const int32_t* __restrict__ cell_of,
                   const uint32_t* __restrict__ offset, const int32_t* __restrict__ cell_ids,
                   uint32_t id_stride,
