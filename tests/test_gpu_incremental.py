@@ -327,3 +327,22 @@ def test_full_batch_2000_steps_forms_equal_and_oracle(psx_mod, oracle):
     rp, rv = _oracle(oracle, scs[k], steps)
     assert_bitwise(as_dtype(got[1][0][offs[k]:offs[k + 1]], "f32"), rp, "scene 80 pos vs oracle")
     assert_bitwise(as_dtype(got[1][1][offs[k]:offs[k + 1]], "f32"), rv, "scene 80 vel vs oracle")
+
+
+def test_hopper_2000_steps_forms_equal(psx_mod):
+    """The 4,194,304-sphere hopper for 2,000 steps with its tuned shape, replayed from prepared graphs:
+    the incremental form's state equals the full form's bit for bit (the full form's hopper steps are
+    pinned to the oracle in test_gpu_configs.py)."""
+    sc = scenes.hopper_settle()
+    steps = 2000
+    got = []
+    for form in (1, 3):
+        with psx_mod.Simulation(sc.pos, sc.vel, sc.radius, sc.mass, sc.params, sc.mesh) as s:
+            _tuned(s)
+            s.set_tuning(use_graph=1, sort_variant=form)
+            s.prepare(steps)
+            s.step(steps)
+            s.synchronize()
+            got.append(s.state())
+    assert_bitwise(got[1][0], got[0][0], "hopper pos, incremental vs full")
+    assert_bitwise(got[1][1], got[0][1], "hopper vel, incremental vs full")
