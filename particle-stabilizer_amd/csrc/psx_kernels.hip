@@ -1395,11 +1395,13 @@ template <class T>
 __device__ __forceinline__ void pair_test2q(const PairSelf<T>& me, const Vec4<T>& c0,
                                             const Vec4<T>& c1, uint32_t q, uint32_t e,
                                             uint32_t self, NearQueue& nq) {
-  const bool m0 = q < e && q != self && pair_near(me, c0);
-  const bool m1 = q + 1 < e && q + 1 != self && pair_near(me, c1);
+  // the particle itself passes the prefilter (distance 0) and is left out where candidates are filed,
+  // so the rounds that find nothing near, most of them, skip that comparison
+  const bool m0 = q < e && pair_near(me, c0);
+  const bool m1 = q + 1 < e && pair_near(me, c1);
   if (m0 || m1) {
-    if (m0) near_push(nq, q);
-    if (m1) near_push(nq, q + 1);
+    if (m0 && q != self) near_push(nq, q);
+    if (m1 && q + 1 != self) near_push(nq, q + 1);
   }
 }
 
