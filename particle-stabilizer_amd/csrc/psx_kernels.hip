@@ -1362,13 +1362,13 @@ __device__ __forceinline__ void pair_test4q(const PairSelf<T>& me, const Vec4<T>
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint32_t j = q + (uint32_t)k;
-    m[k] = (j < e) & (j != self) & pair_near(me, c[k]);
+    m[k] = (j < e) & pair_near(me, c[k]);
     any = any | m[k];
   }
-  if (any) {
+  if (any) {  // the particle itself (distance 0) is left out here, as in pair_test2q
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (m[k]) near_push(nq, q + (uint32_t)k);
+      if (m[k] && q + (uint32_t)k != self) near_push(nq, q + (uint32_t)k);
   }
 }
 
